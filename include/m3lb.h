@@ -181,6 +181,10 @@ int m3lb_set_omega(m3lb_solver* s, double omega, double omega_minus, double omeg
  * Setting them re-encodes the array in place.  Ignored for F32/F64. */
 int m3lb_get_scales(m3lb_solver* s, double* scales_q);
 int m3lb_set_scales(m3lb_solver* s, const double* scales_q);
+/* Re-chooses the 16-bit scales from the current field: per opposite pair the
+ * largest power of two keeping max |g_q| * s_q below 2^14.  Runs after every
+ * initialize; callers may repeat it on a schedule.  No-op for F32/F64. */
+int m3lb_calibrate_scales(m3lb_solver* s);
 
 /* Runs nsteps steps bracketed by HIP events recorded on the handle's stream
  * and returns the elapsed milliseconds.  Used by the benchmark.  No

@@ -1,0 +1,80 @@
+"""Builds libm3lb_hip.so for gfx950 with hipcc.
+
+Twelve instantiation units (one per lattice x collision operator, from
+csrc/inst.hip) and the C-ABI unit compile in parallel into build objects under
+m3lb_amd/_build/, then link into m3lb_amd/libm3lb_hip.so.  Every HIP line
+carries -Xclang -target-feature -Xclang -packed-fp32-ops: hipcc then emits no
+packed FP32 instructions.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libm3lb_hip.so")
+ARCH = "gfx950"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-I", os.path.join(ROOT, "include"), "-Wno-unused-command-line-argument"]
+
+UNITS = [(q, op) for q in (9, 27) for op in range(6)]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return cmd
+
+
+def _sources():
+    return [os.path.join(CSRC, f) for f in os.listdir(CSRC)
+            if f.endswith((".hip", ".hpp", ".h"))] + [os.path.join(ROOT, "include", "m3lb.h")]
+
+
+def _stale(obj, deps):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(force=True, jobs=None, verbose=False):
+    """Compiles every unit (force=True) or only those older than a source."""
+    os.makedirs(OBJ, exist_ok=True)
+    deps = _sources()
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    cmds = []
+    objs = []
+    for q, op in UNITS:
+        o = os.path.join(OBJ, f"inst_q{q}_op{op}.o")
+        objs.append(o)
+        if force or _stale(o, deps):
+            cmds.append([HIPCC] + FLAGS + [f"-DM3LB_Q={q}", f"-DM3LB_OP={op}", "-c",
+                                           os.path.join(CSRC, "inst.hip"), "-o", o])
+    o = os.path.join(OBJ, "solver.o")
+    objs.append(o)
+    if force or _stale(o, deps):
+        cmds.append([HIPCC] + FLAGS + ["-c", os.path.join(CSRC, "solver.hip"), "-o", o])
+    # largest units first
+    cmds.sort(key=lambda c: ("q27" not in c[-1], c[-1]))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for c in ex.map(_run, cmds):
+            if verbose:
+                print("built", c[-1], flush=True)
+    if force or cmds or not os.path.exists(LIB):
+        tmp = LIB + ".tmp"
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+        os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(force="--force" in sys.argv, verbose=True)
+    print(LIB)

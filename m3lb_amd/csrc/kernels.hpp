@@ -1,0 +1,589 @@
+// kernels.hpp -- gfx950 kernels of the esoteric-pull step and its diagnostics.
+//
+// Population layout in HBM: one array, SoA, A[q][n] at base + q*stride + n with
+// n the reference's padded x-fastest node index (src/grid/Domain.hpp:60-62) and
+// stride = n_padded rounded up to 64 so every direction starts 256-B aligned.
+// One thread updates one node; a block covers whole x-rows, so the x-shifted
+// neighbour accesses of a wavefront share cache lines only with wavefronts of
+// the same block.
+#pragma once
+
+#include "lbm_device.hpp"
+
+namespace m3lb {
+
+constexpr int kBlock = 256;
+
+// Domain as the kernels see it: interior extents, halos, padded extents.
+struct Dom {
+  int nx, ny, nz, hx, hy, hz, sx, sy, sz;
+  int px, py, pz;  // periodic flags
+  int np;          // n_padded
+};
+
+struct StepArgs {
+  void* f;
+  uint64_t stride;
+  const uint8_t* flags;  // used when the geometry template flag is set
+  Dom d;
+  int bx_log2;     // threads per x-row = 1 << bx_log2 (64, 128 or 256)
+  int rows;        // ny * nz
+  double omega, omega_m, omega_b;
+  double F[3];
+  void* macro[4];  // rho, ux, uy, uz (compute type), or null
+  float scale[27], inv_scale[27];
+};
+
+template <class K>
+__device__ __forceinline__ Params<typename K::R> params_of(const StepArgs& a) {
+  using R = typename K::R;
+  Params<R> p;
+  p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
+  p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
+  return p;
+}
+
+// Neighbour offsets of an INTERIOR node along each axis (-1, 0, +1 -> index
+// delta), with periodic wrap inside the interior; non-periodic axes have a halo
+// so interior +- 1 is always inside the padded box (Domain.hpp:104-123).
+struct Offs {
+  int xm, xp, ym, yp, zm, zp;
+};
+
+__device__ __forceinline__ Offs offsets(const Dom& d, int x, int y, int z) {
+  Offs o;
+  const int sxy = d.sx * d.sy;
+  o.xm = (d.px && x == 0) ? d.nx - 1 : -1;
+  o.xp = (d.px && x == d.nx - 1) ? -(d.nx - 1) : 1;
+  o.ym = (d.py && y == 0) ? (d.ny - 1) * d.sx : -d.sx;
+  o.yp = (d.py && y == d.ny - 1) ? -(d.ny - 1) * d.sx : d.sx;
+  o.zm = (d.pz && z == 0) ? (d.nz - 1) * sxy : -sxy;
+  o.zp = (d.pz && z == d.nz - 1) ? -(d.nz - 1) * sxy : sxy;
+  return o;
+}
+
+template <int Q, int i>
+__device__ __forceinline__ int nb_off(const Offs& o) {
+  constexpr int cx = Lat<Q>::cx[i], cy = Lat<Q>::cy[i], cz = Lat<Q>::cz[i];
+  int v = 0;
+  if constexpr (cx < 0) v += o.xm;
+  if constexpr (cx > 0) v += o.xp;
+  if constexpr (cy < 0) v += o.ym;
+  if constexpr (cy > 0) v += o.yp;
+  if constexpr (cz < 0) v += o.zm;
+  if constexpr (cz > 0) v += o.zp;
+  return v;
+}
+
+// Element (q, m) of the population array.  The per-direction base is uniform
+// (SGPRs) and the node part a 32-bit byte offset, which maps onto gfx950's
+// global_load/store "saddr + voffset" form: one VGPR addresses every local slot
+// of a node.  Requires n_padded * sizeof(T) < 2^32 (checked at creation).
+template <class T>
+__device__ __forceinline__ const T* qbase(const T* A, uint64_t stride, int q) {
+  return A + uint64_t(q) * stride;
+}
+template <class T>
+__device__ __forceinline__ T ld(const T* base, unsigned node) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     unsigned(node * unsigned(sizeof(T))));
+}
+template <class T>
+__device__ __forceinline__ void st(T* base, unsigned node, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + unsigned(node * unsigned(sizeof(T)))) = v;
+}
+
+// load_rest / load_pair of EsotericPull::Access<P> (EsotericPull.hpp:213-223)
+template <class K, int P>
+__device__ __forceinline__ void load_node(const typename K::T* __restrict__ A, uint64_t stride,
+                                          unsigned n, const Offs& o, const float* inv,
+                                          typename K::R* f) {
+  using C = typename K::C;
+  constexpr int Q = K::Q;
+  f[0] = C::dec(ld(qbase(A, stride, 0), n), inv[0]);
+  sfor<(Q - 1) / 2>([&](auto Pr) {
+    constexpr int i = 2 * Pr + 1;
+    const unsigned nb = unsigned(int(n) + nb_off<Q, i>(o));
+    if constexpr (P == 0) {
+      f[i] = C::dec(ld(qbase(A, stride, i), n), inv[i]);
+      f[i + 1] = C::dec(ld(qbase(A, stride, i + 1), nb), inv[i + 1]);
+    } else {
+      f[i] = C::dec(ld(qbase(A, stride, i + 1), n), inv[i + 1]);
+      f[i + 1] = C::dec(ld(qbase(A, stride, i), nb), inv[i]);
+    }
+  });
+}
+
+// store_rest / store_pair (EsotericPull.hpp:218-224): writes crossed into the
+// very slots the load read.
+template <class K, int P>
+__device__ __forceinline__ void store_node(typename K::T* __restrict__ A, uint64_t stride,
+                                           unsigned n, const Offs& o, const float* s,
+                                           const typename K::R* f) {
+  using C = typename K::C;
+  using T = typename K::T;
+  constexpr int Q = K::Q;
+  auto wb = [&](int q) { return const_cast<T*>(qbase<T>(A, stride, q)); };
+  st(wb(0), n, C::enc(f[0], s[0]));
+  sfor<(Q - 1) / 2>([&](auto Pr) {
+    constexpr int i = 2 * Pr + 1;
+    const unsigned nb = unsigned(int(n) + nb_off<Q, i>(o));
+    if constexpr (P == 0) {
+      st(wb(i + 1), nb, C::enc(f[i], s[i + 1]));
+      st(wb(i), n, C::enc(f[i + 1], s[i]));
+    } else {
+      st(wb(i), nb, C::enc(f[i], s[i]));
+      st(wb(i + 1), n, C::enc(f[i + 1], s[i + 1]));
+    }
+  });
+}
+
+// run_step<P, StoreMacro> (FluidSolver.hpp:712-1038), Fluid and Solid paths:
+// solid and excluded nodes do nothing (implicit bounce-back, :759-761).
+template <class K, int P, bool GEOM>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  const int bx = 1 << a.bx_log2;
+  const int tx = threadIdx.x & (bx - 1);
+  const int row = blockIdx.x * (kBlock >> a.bx_log2) + (threadIdx.x >> a.bx_log2);
+  if (row >= a.rows) return;
+  const Dom& d = a.d;
+  const int y = row % d.ny;
+  const int z = row / d.ny;
+  const Params<R> p = params_of<K>(a);
+  T* __restrict__ A = static_cast<T*>(a.f);
+  const unsigned rowbase = unsigned(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + d.hx);
+  for (int x = tx; x < d.nx; x += bx) {
+    const unsigned n = rowbase + unsigned(x);
+    if constexpr (GEOM) {
+      if (a.flags[n] != M3LB_FLUID) continue;
+    }
+    const Offs o = offsets(d, x, y, z);
+    R f[K::Q];
+    load_node<K, P>(A, a.stride, n, o, a.inv_scale, f);
+    const Macro<R> m = macroscopic<K>(f, p);
+    collide<K>(f, m, p);
+    store_node<K, P>(A, a.stride, n, o, a.scale, f);
+    if (a.macro[0]) {
+      static_cast<R*>(a.macro[0])[n] = density<K>(m);
+      static_cast<R*>(a.macro[1])[n] = m.ux;
+      static_cast<R*>(a.macro[2])[n] = m.uy;
+      static_cast<R*>(a.macro[3])[n] = m.uz;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Seeding.  The reference's scatter<P> (FluidSolver.hpp:1231-1241) walks the
+// padded box in ascending node order and writes population i of node n into
+// the slot load() reads, with neighbours clamped at non-periodic faces; where
+// clamping makes two nodes write one slot the later (higher-indexed) one is
+// what remains.  Here every slot is visited once and takes the value of its
+// highest-indexed writer, which is that same result with no write conflict.
+// ---------------------------------------------------------------------------
+enum SeedMode { kSeedUniform = 0, kSeedField = 1, kSeedArray = 2 };
+
+struct SeedArgs {
+  void* f;
+  uint64_t stride;
+  const uint8_t* flags;
+  Dom d;
+  int parity;
+  int mode;
+  double rho0, u0[3];
+  const double* field[4];  // rho, ux, uy, uz (padded), kSeedField
+  const double* arr;       // (n, q) logical populations, kSeedArray
+  float scale[27];
+};
+
+// Along one axis: the largest coordinate v with shift(v + e) == m, or -1.
+__device__ __forceinline__ int writer_axis(int m, int e, int n, int s, int per) {
+  if (e == 0) return m;
+  if (per) {
+    int v = m - e;
+    if (v < 0) v += n;
+    if (v >= n) v -= n;
+    return v;
+  }
+  int best = -1;
+  const int v = m - e;
+  if (v >= 0 && v < s) best = v;
+  if ((e < 0 && m == 0) || (e > 0 && m == s - 1)) best = best > m ? best : m;
+  return best;
+}
+
+template <class K>
+__device__ __forceinline__ void seed_value_of(const SeedArgs& a, int n, typename K::R* v) {
+  using R = typename K::R;
+  const bool fluid = a.flags[n] == M3LB_FLUID;
+  if (a.mode == kSeedUniform) {
+    if (fluid) seed<K>(v, R(a.rho0), R(a.u0[0]), R(a.u0[1]), R(a.u0[2]));
+    else seed<K>(v, R(a.rho0), R(0), R(0), R(0));
+  } else {
+    if (fluid)
+      seed<K>(v, R(a.field[0][n]), R(a.field[1][n]), R(a.field[2][n]), R(a.field[3][n]));
+    else seed<K>(v, R(1), R(0), R(0), R(0));
+  }
+}
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_seed(SeedArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  using C = typename K::C;
+  constexpr int Q = K::Q;
+  const int m = blockIdx.x * kBlock + threadIdx.x;
+  const Dom& d = a.d;
+  if (m >= d.np) return;
+  const int mx = m % d.sx, my = (m / d.sx) % d.sy, mz = m / (d.sx * d.sy);
+  T* A = static_cast<T*>(a.f);
+  R own[Q];
+  if (a.mode != kSeedArray) seed_value_of<K>(a, m, own);
+  sfor<Q>([&](auto Qs) {
+    constexpr int q = Qs;
+    // direction i written into slot q, and the displacement e with
+    // nbr(writer, e) == m for remote slots (EsotericPull.hpp:230-237)
+    int i = q, ex = 0, ey = 0, ez = 0;
+    bool remote = false;
+    if constexpr (q != 0) {
+      if (a.parity == 0) {
+        if constexpr ((q & 1) == 0) {
+          remote = true;
+          ex = Lat<Q>::cx[q - 1]; ey = Lat<Q>::cy[q - 1]; ez = Lat<Q>::cz[q - 1];
+        }
+      } else {
+        if constexpr ((q & 1) == 0) {
+          i = q - 1;
+        } else {
+          i = q + 1;
+          remote = true;
+          ex = Lat<Q>::cx[q]; ey = Lat<Q>::cy[q]; ez = Lat<Q>::cz[q];
+        }
+      }
+    }
+    int w = m;
+    if (remote) {
+      const int wx = writer_axis(mx, ex, d.nx, d.sx, d.px);
+      const int wy = writer_axis(my, ey, d.ny, d.sy, d.py);
+      const int wz = writer_axis(mz, ez, d.nz, d.sz, d.pz);
+      w = (wx < 0 || wy < 0 || wz < 0) ? -1 : (wz * d.sy + wy) * d.sx + wx;
+    }
+    if (w < 0) return;  // no node writes this slot: it keeps its value
+    R val;
+    if (a.mode == kSeedArray) {
+      val = R(a.arr[uint64_t(w) * Q + i]);
+    } else if (w == m) {
+      val = R(0);
+      sfor<Q>([&](auto J) { if (int(J) == i) val = own[int(J)]; });
+    } else {
+      R other[Q];
+      seed_value_of<K>(a, w, other);
+      val = R(0);
+      sfor<Q>([&](auto J) { if (int(J) == i) val = other[int(J)]; });
+    }
+    A[uint64_t(q) * a.stride + unsigned(m)] = C::enc(val, a.scale[q]);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Logical readout: gather<P> (FluidSolver.hpp:1243-1253) through
+// Access<P>::load (EsotericPull.hpp:191-198) with the reference's neighbour
+// rule, clamping included, for every padded node.
+// ---------------------------------------------------------------------------
+struct ReadArgs {
+  const void* f;
+  uint64_t stride;
+  const uint8_t* flags;
+  Dom d;
+  int parity;
+  double* out;       // gather: (n, q); macro: rho, ux, uy, uz planes
+  double* out2[4];
+  float inv_scale[27];
+  double omega, omega_m, omega_b, F[3];
+};
+
+__device__ __forceinline__ int shift_axis(int v, int h, int n, int s, int per) {
+  if (per) {
+    if (v < h) return v + n;
+    if (v >= h + n) return v - n;
+    return v;
+  }
+  if (v < 0) return 0;
+  if (v >= s) return s - 1;
+  return v;
+}
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_gather(ReadArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  using C = typename K::C;
+  constexpr int Q = K::Q;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  const Dom& d = a.d;
+  if (n >= d.np) return;
+  const int x = n % d.sx, y = (n / d.sx) % d.sy, z = n / (d.sx * d.sy);
+  const T* A = static_cast<const T*>(a.f);
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    int node = n, slot = i;
+    if constexpr (i != 0) {
+      if constexpr ((i & 1) == 0) {  // read from the odd partner's neighbour
+        constexpr int j = i - 1;
+        const int nx_ = shift_axis(x + Lat<Q>::cx[j], d.hx, d.nx, d.sx, d.px);
+        const int ny_ = shift_axis(y + Lat<Q>::cy[j], d.hy, d.ny, d.sy, d.py);
+        const int nz_ = shift_axis(z + Lat<Q>::cz[j], d.hz, d.nz, d.sz, d.pz);
+        node = (nz_ * d.sy + ny_) * d.sx + nx_;
+        slot = a.parity == 0 ? i : i - 1;
+      } else {
+        slot = a.parity == 0 ? i : i + 1;
+      }
+    }
+    const R v = C::dec(A[uint64_t(slot) * a.stride + unsigned(node)], a.inv_scale[slot]);
+    a.out[uint64_t(n) * Q + i] = double(v);
+  });
+}
+
+// Raw slot image (EsotericPull::storage()), (n, q) ordered.
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_storage(ReadArgs a) {
+  using T = typename K::T;
+  using C = typename K::C;
+  constexpr int Q = K::Q;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= a.d.np) return;
+  const T* A = static_cast<const T*>(a.f);
+  sfor<Q>([&](auto I) {
+    constexpr int q = I;
+    a.out[uint64_t(n) * Q + q] = double(C::dec(A[uint64_t(q) * a.stride + unsigned(n)],
+                                               a.inv_scale[q]));
+  });
+}
+
+// macro_kernel<P> fluid path (FluidSolver.hpp:1053-1056, 1192-1199): rho and u
+// at the parity the next step reads; 0 on every other node.
+template <class K, int P>
+__device__ __forceinline__ void interior_coords(const Dom& d, int n, int& x, int& y, int& z) {
+  x = n % d.sx - d.hx;
+  y = (n / d.sx) % d.sy - d.hy;
+  z = n / (d.sx * d.sy) - d.hz;
+}
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_macro(ReadArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  const Dom& d = a.d;
+  if (n >= d.np) return;
+  double r = 0, ux = 0, uy = 0, uz = 0;
+  if (a.flags[n] == M3LB_FLUID) {
+    int x, y, z;
+    interior_coords<K, 0>(d, n, x, y, z);
+    const Offs o = offsets(d, x, y, z);
+    Params<R> p;
+    p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
+    p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
+    R f[K::Q];
+    const T* A = static_cast<const T*>(a.f);
+    if (a.parity == 0) load_node<K, 0>(A, a.stride, unsigned(n), o, a.inv_scale, f);
+    else load_node<K, 1>(A, a.stride, unsigned(n), o, a.inv_scale, f);
+    const Macro<R> m = macroscopic<K>(f, p);
+    r = double(density<K>(m)); ux = double(m.ux); uy = double(m.uy); uz = double(m.uz);
+  }
+  if (a.out2[0]) a.out2[0][n] = r;
+  if (a.out2[1]) a.out2[1][n] = ux;
+  if (a.out2[2]) a.out2[2][n] = uy;
+  if (a.out2[3]) a.out2[3][n] = uz;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.  Wave64 butterfly, then LDS across the block's waves; one
+// partial per block, summed on the host in block order (deterministic).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
+  __shared__ double lds[NV][kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) lds[k][wv] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0;
+      for (int w = 0; w < kBlock / 64; ++w) s += lds[k][w];
+      out[uint64_t(blockIdx.x) * NV + k] = s;
+    }
+  }
+}
+
+// total_mass / total_momentum (reduce<P>, FluidSolver.hpp:1205-1229): per
+// fluid node rho, rho ux, rho uy, rho uz, all four at once.
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_reduce_mass(ReadArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  const Dom& d = a.d;
+  double acc[4] = {0, 0, 0, 0};
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < d.np; n += gridDim.x * kBlock) {
+    if (a.flags[n] != M3LB_FLUID) continue;
+    int x, y, z;
+    interior_coords<K, 0>(d, n, x, y, z);
+    const Offs o = offsets(d, x, y, z);
+    Params<R> p;
+    p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
+    p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
+    R f[K::Q];
+    const T* A = static_cast<const T*>(a.f);
+    if (a.parity == 0) load_node<K, 0>(A, a.stride, unsigned(n), o, a.inv_scale, f);
+    else load_node<K, 1>(A, a.stride, unsigned(n), o, a.inv_scale, f);
+    const Macro<R> m = macroscopic<K>(f, p);
+    const R r = density<K>(m);
+    acc[0] += double(r);
+    acc[1] += double(r * m.ux);
+    acc[2] += double(r * m.uy);
+    acc[3] += double(r * m.uz);
+  }
+  block_sum<4>(acc, a.out);
+}
+
+// Exact sum of every slot: value * 2^100 truncated toward zero, summed as a
+// 128-bit two's-complement integer (two 64-bit limbs with carry).
+struct U128 {
+  uint64_t hi, lo;
+};
+
+__device__ __forceinline__ U128 add128(U128 a, U128 b) {
+  U128 r;
+  r.lo = a.lo + b.lo;
+  r.hi = a.hi + b.hi + (r.lo < a.lo ? 1 : 0);
+  return r;
+}
+
+__device__ __forceinline__ U128 fixed100(double v) {
+  U128 r{0, 0};
+  if (v == 0.0 || !isfinite(v)) return r;
+  const uint64_t bits = __double_as_longlong(v);
+  const int ebits = int((bits >> 52) & 0x7ff);
+  uint64_t mant = bits & 0xfffffffffffffull;
+  int e;  // |v| = mant * 2^e
+  if (ebits == 0) {
+    e = -1074;
+  } else {
+    mant |= (1ull << 52);
+    e = ebits - 1075;
+  }
+  const int sh = e + 100;
+  if (sh >= 0) {
+    if (sh + 53 > 126) {
+      r.hi = 1ull << 62;
+    } else if (sh >= 64) {
+      r.hi = mant << (sh - 64);
+    } else if (sh == 0) {
+      r.lo = mant;
+    } else {
+      r.lo = mant << sh;
+      r.hi = mant >> (64 - sh);
+    }
+  } else if (sh > -64) {
+    r.lo = mant >> (-sh);
+  }
+  if (v < 0) {
+    r.lo = ~r.lo;
+    r.hi = ~r.hi;
+    r = add128(r, U128{0, 1});
+  }
+  return r;
+}
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_sum_exact(ReadArgs a) {
+  using T = typename K::T;
+  using C = typename K::C;
+  constexpr int Q = K::Q;
+  const T* A = static_cast<const T*>(a.f);
+  U128 acc{0, 0};
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < a.d.np; n += gridDim.x * kBlock) {
+    sfor<Q>([&](auto I) {
+      constexpr int q = I;
+      acc = add128(acc, fixed100(double(C::dec(A[uint64_t(q) * a.stride + unsigned(n)],
+                                                a.inv_scale[q]))));
+    });
+  }
+  __shared__ uint64_t lh[kBlock], ll[kBlock];
+  lh[threadIdx.x] = acc.hi;
+  ll[threadIdx.x] = acc.lo;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    U128 s{0, 0};
+    for (int k = 0; k < kBlock; ++k) s = add128(s, U128{lh[k], ll[k]});
+    uint64_t* o = reinterpret_cast<uint64_t*>(a.out);
+    o[2 * blockIdx.x] = s.hi;
+    o[2 * blockIdx.x + 1] = s.lo;
+  }
+}
+
+// Per-slot maximum |stored value| (decoded), for the 16-bit scale choice.
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_absmax(ReadArgs a) {
+  using T = typename K::T;
+  using C = typename K::C;
+  constexpr int Q = K::Q;
+  const T* A = static_cast<const T*>(a.f);
+  double mx[Q];
+  sfor<Q>([&](auto I) { mx[int(I)] = 0.0; });
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < a.d.np; n += gridDim.x * kBlock) {
+    sfor<Q>([&](auto I) {
+      constexpr int q = I;
+      const double v = fabs(double(C::dec(A[uint64_t(q) * a.stride + unsigned(n)],
+                                          a.inv_scale[q])));
+      mx[q] = v > mx[q] ? v : mx[q];
+    });
+  }
+  __shared__ double lds[kBlock];
+  sfor<Q>([&](auto I) {
+    constexpr int q = I;
+    lds[threadIdx.x] = mx[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = 0;
+      for (int k = 0; k < kBlock; ++k) m = lds[k] > m ? lds[k] : m;
+      a.out[uint64_t(blockIdx.x) * Q + q] = m;
+    }
+    __syncthreads();
+  });
+}
+
+// Re-encode 16-bit storage from old to new scales (powers of two: exact unless
+// the new code over- or underflows).
+struct Ratio {
+  float r[27];
+};
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_rescale(void* f, uint64_t stride, int np,
+                                                    Ratio ratio) {
+  using T = typename K::T;
+  constexpr int Q = K::Q;
+  T* A = static_cast<T*>(f);
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < np; n += gridDim.x * kBlock) {
+    sfor<Q>([&](auto I) {
+      constexpr int q = I;
+      T& c = A[uint64_t(q) * stride + unsigned(n)];
+      c = T(float(c) * ratio.r[q]);
+    });
+  }
+}
+
+}  // namespace m3lb

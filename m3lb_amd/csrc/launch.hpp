@@ -1,0 +1,78 @@
+// launch.hpp -- host launchers of one kernel configuration, gathered into a
+// table the solver selects at creation from the runtime config.  This table is
+// the only runtime dispatch: every kernel below is specialised at compile time
+// on lattice, operator, storage, dtype, forcing, parity and geometry, as the
+// reference's FluidSolver / collision templates are (FluidSolver.hpp:34-35,
+// 582-591).
+#pragma once
+
+#include "kernels.hpp"
+
+namespace m3lb {
+
+struct KernelSet {
+  void (*step)(const StepArgs&, int parity, bool geom, unsigned grid, hipStream_t);
+  void (*seed)(const SeedArgs&, hipStream_t);
+  void (*gather)(const ReadArgs&, hipStream_t);
+  void (*storage)(const ReadArgs&, hipStream_t);
+  void (*macro)(const ReadArgs&, hipStream_t);
+  void (*reduce_mass)(const ReadArgs&, unsigned grid, hipStream_t);
+  void (*sum_exact)(const ReadArgs&, unsigned grid, hipStream_t);
+  void (*absmax)(const ReadArgs&, unsigned grid, hipStream_t);
+  void (*rescale)(void* f, uint64_t stride, int np, const Ratio&, unsigned grid, hipStream_t);
+  int elem_bytes;    // bytes per stored population
+  int real_bytes;    // bytes of the compute type
+};
+
+inline unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+template <class K>
+struct Launch {
+  static void step(const StepArgs& a, int parity, bool geom, unsigned grid, hipStream_t s) {
+    if (parity == 0) {
+      if (geom) k_step<K, 0, true><<<grid, kBlock, 0, s>>>(a);
+      else k_step<K, 0, false><<<grid, kBlock, 0, s>>>(a);
+    } else {
+      if (geom) k_step<K, 1, true><<<grid, kBlock, 0, s>>>(a);
+      else k_step<K, 1, false><<<grid, kBlock, 0, s>>>(a);
+    }
+  }
+  static void seed(const SeedArgs& a, hipStream_t s) {
+    k_seed<K><<<blocks_for(a.d.np), kBlock, 0, s>>>(a);
+  }
+  static void gather(const ReadArgs& a, hipStream_t s) {
+    k_gather<K><<<blocks_for(a.d.np), kBlock, 0, s>>>(a);
+  }
+  static void storage(const ReadArgs& a, hipStream_t s) {
+    k_storage<K><<<blocks_for(a.d.np), kBlock, 0, s>>>(a);
+  }
+  static void macro(const ReadArgs& a, hipStream_t s) {
+    k_macro<K><<<blocks_for(a.d.np), kBlock, 0, s>>>(a);
+  }
+  static void reduce_mass(const ReadArgs& a, unsigned grid, hipStream_t s) {
+    k_reduce_mass<K><<<grid, kBlock, 0, s>>>(a);
+  }
+  static void sum_exact(const ReadArgs& a, unsigned grid, hipStream_t s) {
+    k_sum_exact<K><<<grid, kBlock, 0, s>>>(a);
+  }
+  static void absmax(const ReadArgs& a, unsigned grid, hipStream_t s) {
+    k_absmax<K><<<grid, kBlock, 0, s>>>(a);
+  }
+  static void rescale(void* f, uint64_t stride, int np, const Ratio& r, unsigned grid,
+                      hipStream_t s) {
+    if constexpr (K::C::scaled) k_rescale<K><<<grid, kBlock, 0, s>>>(f, stride, np, r);
+  }
+  static const KernelSet* table() {
+    static const KernelSet t = {step,     seed,      gather, storage,
+                                macro,    reduce_mass, sum_exact, absmax,
+                                rescale,  int(sizeof(typename K::T)), int(sizeof(typename K::R))};
+    return &t;
+  }
+};
+
+}  // namespace m3lb
+
+// One instantiation unit per (lattice, operator); each exports this lookup.
+#define M3LB_LOOKUP_NAME(Q, OP) m3lb_kernels_q##Q##_op##OP
+#define M3LB_DECLARE_LOOKUP(Q, OP) \
+  const m3lb::KernelSet* M3LB_LOOKUP_NAME(Q, OP)(int storage, int dtype, int force)

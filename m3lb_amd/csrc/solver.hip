@@ -1,0 +1,699 @@
+// solver.hip -- the C-ABI of include/m3lb.h: a handle owning one in-place
+// population array on the current HIP device, its flags and diagnostics
+// buffers, and one HIP stream that orders every call on the handle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+
+using m3lb::KernelSet;
+
+#define M3LB_X(Q, OP) M3LB_DECLARE_LOOKUP(Q, OP);
+M3LB_X(9, 0) M3LB_X(9, 1) M3LB_X(9, 2) M3LB_X(9, 3) M3LB_X(9, 4) M3LB_X(9, 5)
+M3LB_X(27, 0) M3LB_X(27, 1) M3LB_X(27, 2) M3LB_X(27, 3) M3LB_X(27, 4) M3LB_X(27, 5)
+#undef M3LB_X
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_OK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(M3LB_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+const KernelSet* lookup(int q, int op, int st, int dt, int force) {
+  typedef const KernelSet* (*Fn)(int, int, int);
+  static const Fn q9[6] = {m3lb_kernels_q9_op0, m3lb_kernels_q9_op1, m3lb_kernels_q9_op2,
+                           m3lb_kernels_q9_op3, m3lb_kernels_q9_op4, m3lb_kernels_q9_op5};
+  static const Fn q27[6] = {m3lb_kernels_q27_op0, m3lb_kernels_q27_op1, m3lb_kernels_q27_op2,
+                            m3lb_kernels_q27_op3, m3lb_kernels_q27_op4, m3lb_kernels_q27_op5};
+  if (op < 0 || op > 5) return nullptr;
+  return (q == 9 ? q9 : q27)[op](st, dt, force);
+}
+
+void domain_of(const m3lb_config& c, int64_t out[10]) {
+  const int64_t n[3] = {c.nx, c.ny, c.nz};
+  int64_t h[3];
+  for (int a = 0; a < 3; ++a) h[a] = (n[a] <= 1 || c.periodic[a]) ? 0 : 1;  // Domain.hpp:104-108
+  for (int a = 0; a < 3; ++a) { out[a] = n[a]; out[3 + a] = h[a]; out[6 + a] = n[a] + 2 * h[a]; }
+  out[9] = out[6] * out[7] * out[8];
+}
+
+int validate(const m3lb_config* c) {
+  if (!c) return fail(M3LB_EINVAL, "config is null");
+  if (c->lattice != M3LB_D2Q9 && c->lattice != M3LB_D3Q27)
+    return fail(M3LB_EINVAL, "lattice must be M3LB_D2Q9 or M3LB_D3Q27");
+  if (c->op < M3LB_OP_NULL || c->op > M3LB_OP_CM) return fail(M3LB_EINVAL, "unknown operator");
+  if (c->storage != M3LB_RAW && c->storage != M3LB_SHIFTED)
+    return fail(M3LB_EINVAL, "storage must be M3LB_RAW or M3LB_SHIFTED");
+  if (c->dtype < M3LB_F64 || c->dtype > M3LB_BF16) return fail(M3LB_EINVAL, "unknown dtype");
+  if ((c->dtype == M3LB_F16 || c->dtype == M3LB_BF16) &&
+      (c->storage != M3LB_SHIFTED || c->op == M3LB_OP_NULL))
+    return fail(M3LB_EINVAL, "16-bit storage holds shifted populations (g = f - w) only");
+  if (c->forcing != M3LB_FORCE_NONE && c->forcing != M3LB_FORCE_GUO)
+    return fail(M3LB_EINVAL, "forcing must be M3LB_FORCE_NONE or M3LB_FORCE_GUO");
+  if (c->reserved != 0) return fail(M3LB_EINVAL, "reserved must be 0");
+  if (c->nx < 1 || c->ny < 1 || c->nz < 1) return fail(M3LB_EINVAL, "extents must be >= 1");
+  if (c->lattice == M3LB_D2Q9 && c->nz != 1) return fail(M3LB_EINVAL, "D2Q9 needs nz == 1");
+  if (c->op == M3LB_OP_NULL && c->forcing != M3LB_FORCE_NONE)
+    return fail(M3LB_EINVAL, "the identity collision takes no force");
+  if (!(std::isfinite(c->omega) && std::isfinite(c->omega_minus) && std::isfinite(c->omega_bulk)))
+    return fail(M3LB_EINVAL, "relaxation rates must be finite");
+  return M3LB_OK;
+}
+
+}  // namespace
+
+struct m3lb_solver {
+  m3lb_config cfg;
+  int Q = 0;
+  int64_t dom[10];
+  m3lb::Dom d;
+  const KernelSet* ks = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* f = nullptr;
+  uint64_t stride = 0;  // elements per direction
+  uint8_t* flags_d = nullptr;
+  std::vector<uint8_t> flags_h;
+  bool geometry = false;  // any interior node not Fluid
+  void* macro_d[4] = {nullptr, nullptr, nullptr, nullptr};  // compute type, StoreMacro
+  bool macro_valid = false;
+  double* partial_d = nullptr;  // per-block partials
+  size_t partial_n = 0;
+  uint64_t t = 0;
+  float scale[27];
+  float inv_scale[27];
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+unsigned reduce_grid(const m3lb_solver* s) {
+  return std::max(1u, std::min(4096u, m3lb::blocks_for(s->d.np)));
+}
+
+int ensure_partials(m3lb_solver* s, size_t n_doubles) {
+  if (s->partial_n >= n_doubles) return M3LB_OK;
+  if (s->partial_d) HIP_OK(hipFree(s->partial_d));
+  s->partial_d = nullptr;
+  HIP_OK(hipMalloc(&s->partial_d, n_doubles * sizeof(double)));
+  s->partial_n = n_doubles;
+  return M3LB_OK;
+}
+
+m3lb::StepArgs step_args(const m3lb_solver* s) {
+  m3lb::StepArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.f = s->f;
+  a.stride = s->stride;
+  a.flags = s->flags_d;
+  a.d = s->d;
+  const int nx = s->d.nx;
+  a.bx_log2 = nx > 128 ? 8 : (nx > 64 ? 7 : 6);
+  a.rows = s->d.ny * s->d.nz;
+  a.omega = s->cfg.omega;
+  a.omega_m = s->cfg.omega_minus;
+  a.omega_b = s->cfg.omega_bulk;
+  for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
+  for (int q = 0; q < 27; ++q) { a.scale[q] = s->scale[q]; a.inv_scale[q] = s->inv_scale[q]; }
+  return a;
+}
+
+m3lb::ReadArgs read_args(const m3lb_solver* s) {
+  m3lb::ReadArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.f = s->f;
+  a.stride = s->stride;
+  a.flags = s->flags_d;
+  a.d = s->d;
+  a.parity = int(s->t % 2);
+  a.omega = s->cfg.omega;
+  a.omega_m = s->cfg.omega_minus;
+  a.omega_b = s->cfg.omega_bulk;
+  for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
+  for (int q = 0; q < 27; ++q) a.inv_scale[q] = s->inv_scale[q];
+  return a;
+}
+
+m3lb::SeedArgs seed_args(const m3lb_solver* s) {
+  m3lb::SeedArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.f = s->f;
+  a.stride = s->stride;
+  a.flags = s->flags_d;
+  a.d = s->d;
+  a.parity = int(s->t % 2);
+  for (int q = 0; q < 27; ++q) a.scale[q] = s->scale[q];
+  return a;
+}
+
+bool is16(const m3lb_solver* s) {
+  return s->cfg.dtype == M3LB_F16 || s->cfg.dtype == M3LB_BF16;
+}
+
+// Provisional 16-bit scales before the first seeding: 2^k with |g_q| <= w_q/2
+// mapped below 2^14 (FP16's largest finite code is 65504).
+void provisional_scales(m3lb_solver* s) {
+  static const int wn9[9] = {16, 4, 4, 4, 4, 1, 1, 1, 1};
+  static const int wn27[27] = {64, 16, 16, 16, 16, 16, 16, 4, 4, 4, 4, 4, 4, 4,
+                               4,  4,  4,  4,  4,  1,  1,  1, 1, 1, 1, 1, 1};
+  for (int q = 0; q < 27; ++q) { s->scale[q] = 1.0f; s->inv_scale[q] = 1.0f; }
+  if (!is16(s)) return;
+  for (int q = 0; q < s->Q; ++q) {
+    const double w = s->Q == 9 ? wn9[q] / 36.0 : wn27[q] / 216.0;
+    const int k = int(std::floor(std::log2(16384.0 / (0.5 * w))));
+    s->scale[q] = std::ldexp(1.0f, k);
+    s->inv_scale[q] = std::ldexp(1.0f, -k);
+  }
+}
+
+int launch_ok() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(M3LB_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return M3LB_OK;
+}
+
+int sync(m3lb_solver* s) {
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return M3LB_OK;
+}
+
+int calibrate(m3lb_solver* s);
+
+}  // namespace
+
+extern "C" {
+
+const char* m3lb_last_error(void) { return g_err.c_str(); }
+
+int m3lb_domain(const m3lb_config* cfg, int64_t out[10]) {
+  int rc = validate(cfg);
+  if (rc) return rc;
+  domain_of(*cfg, out);
+  return M3LB_OK;
+}
+
+int m3lb_create(const m3lb_config* cfg, m3lb_solver** out) {
+  if (!out) return fail(M3LB_EINVAL, "out is null");
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc) return rc;
+  const KernelSet* ks = lookup(cfg->lattice, cfg->op, cfg->storage, cfg->dtype, cfg->forcing);
+  if (!ks) return fail(M3LB_EINVAL, "unsupported lattice/operator/storage/dtype combination");
+  int64_t dom[10];
+  domain_of(*cfg, dom);
+  const int64_t np = dom[9];
+  if (np >= (int64_t(1) << 31))
+    return fail(M3LB_EINVAL, "n_padded exceeds the reference's int32 node index");
+  if (np * int64_t(ks->elem_bytes) >= (int64_t(1) << 32))
+    return fail(M3LB_EINVAL, "one direction of the population array exceeds 4 GiB");
+
+  m3lb_solver* s = new m3lb_solver();
+  s->cfg = *cfg;
+  s->Q = cfg->lattice;
+  std::memcpy(s->dom, dom, sizeof dom);
+  s->d.nx = int(dom[0]); s->d.ny = int(dom[1]); s->d.nz = int(dom[2]);
+  s->d.hx = int(dom[3]); s->d.hy = int(dom[4]); s->d.hz = int(dom[5]);
+  s->d.sx = int(dom[6]); s->d.sy = int(dom[7]); s->d.sz = int(dom[8]);
+  s->d.px = cfg->periodic[0] ? 1 : 0;
+  s->d.py = cfg->periodic[1] ? 1 : 0;
+  s->d.pz = cfg->periodic[2] ? 1 : 0;
+  s->d.np = int(np);
+  s->ks = ks;
+  s->stride = (uint64_t(np) + 63) / 64 * 64;
+  provisional_scales(s);
+
+  auto bail = [&](int code) {
+    m3lb_destroy(s);
+    return code;
+  };
+  if (hipGetDevice(&s->device) != hipSuccess) return bail(fail(M3LB_EDEVICE, "no HIP device"));
+  if (hipStreamCreate(&s->stream) != hipSuccess) return bail(fail(M3LB_EDEVICE, "hipStreamCreate failed"));
+  if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return bail(fail(M3LB_EDEVICE, "hipEventCreate failed"));
+  const size_t fbytes = s->stride * size_t(s->Q) * size_t(ks->elem_bytes);
+  if (hipMalloc(&s->f, fbytes) != hipSuccess)
+    return bail(fail(M3LB_ENOMEM, "population array: hipMalloc of " + std::to_string(fbytes) + " bytes failed"));
+  // Kokkos Views are zero-initialised; so is this.
+  if (hipMemsetAsync(s->f, 0, fbytes, s->stream) != hipSuccess)
+    return bail(fail(M3LB_EDEVICE, "hipMemsetAsync failed"));
+  if (hipMalloc(&s->flags_d, size_t(np)) != hipSuccess)
+    return bail(fail(M3LB_ENOMEM, "flags: hipMalloc failed"));
+  // Interior Fluid, halo Excluded (FluidSolver.hpp:59-67).
+  s->flags_h.assign(size_t(np), uint8_t(M3LB_EXCLUDED));
+  for (int z = 0; z < s->d.nz; ++z)
+    for (int y = 0; y < s->d.ny; ++y)
+      for (int x = 0; x < s->d.nx; ++x)
+        s->flags_h[size_t(((z + s->d.hz) * s->d.sy + (y + s->d.hy)) * s->d.sx + x + s->d.hx)] = M3LB_FLUID;
+  if (hipMemcpyAsync(s->flags_d, s->flags_h.data(), size_t(np), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    return bail(fail(M3LB_EDEVICE, "flags upload failed"));
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(M3LB_EDEVICE, "sync failed"));
+  *out = s;
+  return M3LB_OK;
+}
+
+int m3lb_destroy(m3lb_solver* s) {
+  if (!s) return M3LB_OK;
+  if (s->stream) hipStreamSynchronize(s->stream);
+  if (s->f) hipFree(s->f);
+  if (s->flags_d) hipFree(s->flags_d);
+  for (void* p : s->macro_d)
+    if (p) hipFree(p);
+  if (s->partial_d) hipFree(s->partial_d);
+  if (s->ev0) hipEventDestroy(s->ev0);
+  if (s->ev1) hipEventDestroy(s->ev1);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return M3LB_OK;
+}
+
+int m3lb_set_geometry(m3lb_solver* s, const uint8_t* fl) {
+  if (!s || !fl) return fail(M3LB_EINVAL, "null argument");
+  const m3lb::Dom& d = s->d;
+  bool geom = false;
+  size_t k = 0;
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x, ++k) {
+        const uint8_t v = fl[k];
+        if (v > M3LB_EXCLUDED) return fail(M3LB_EINVAL, "flag values are Fluid(0), Solid(1), Excluded(2)");
+        s->flags_h[size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx)] = v;
+        geom = geom || v != M3LB_FLUID;
+      }
+  s->geometry = geom;
+  HIP_OK(hipMemcpyAsync(s->flags_d, s->flags_h.data(), s->flags_h.size(), hipMemcpyHostToDevice, s->stream));
+  return sync(s);
+}
+
+int m3lb_initialize(m3lb_solver* s, double rho0, double ux, double uy, double uz) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  s->t = 0;
+  s->macro_valid = false;
+  provisional_scales(s);
+  m3lb::SeedArgs a = seed_args(s);
+  a.mode = m3lb::kSeedUniform;
+  a.rho0 = rho0;
+  a.u0[0] = ux; a.u0[1] = uy; a.u0[2] = uz;
+  s->ks->seed(a, s->stream);
+  int rc = launch_ok();
+  if (rc) return rc;
+  rc = sync(s);
+  if (rc) return rc;
+  return is16(s) ? calibrate(s) : M3LB_OK;
+}
+
+int m3lb_initialize_field(m3lb_solver* s, const double* rho, const double* ux, const double* uy,
+                          const double* uz) {
+  if (!s || !rho || !ux || !uy || !uz) return fail(M3LB_EINVAL, "null argument");
+  s->t = 0;
+  s->macro_valid = false;
+  provisional_scales(s);
+  const size_t np = size_t(s->d.np);
+  double* buf = nullptr;
+  HIP_OK(hipMalloc(&buf, 4 * np * sizeof(double)));
+  const double* src[4] = {rho, ux, uy, uz};
+  for (int k = 0; k < 4; ++k)
+    HIP_OK(hipMemcpyAsync(buf + k * np, src[k], np * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  m3lb::SeedArgs a = seed_args(s);
+  a.mode = m3lb::kSeedField;
+  for (int k = 0; k < 4; ++k) a.field[k] = buf + k * np;
+  s->ks->seed(a, s->stream);
+  int rc = launch_ok();
+  hipError_t e = hipStreamSynchronize(s->stream);
+  hipFree(buf);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(M3LB_EDEVICE, hipGetErrorString(e));
+  return is16(s) ? calibrate(s) : M3LB_OK;
+}
+
+int m3lb_seed_populations(m3lb_solver* s, const double* in) {
+  if (!s || !in) return fail(M3LB_EINVAL, "null argument");
+  s->macro_valid = false;
+  const size_t n = size_t(s->d.np) * size_t(s->Q);
+  double* buf = nullptr;
+  HIP_OK(hipMalloc(&buf, n * sizeof(double)));
+  HIP_OK(hipMemcpyAsync(buf, in, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  m3lb::SeedArgs a = seed_args(s);
+  a.mode = m3lb::kSeedArray;
+  a.arr = buf;
+  s->ks->seed(a, s->stream);
+  int rc = launch_ok();
+  hipError_t e = hipStreamSynchronize(s->stream);
+  hipFree(buf);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(M3LB_EDEVICE, hipGetErrorString(e));
+  return M3LB_OK;
+}
+
+int m3lb_step(m3lb_solver* s, int32_t nsteps, int32_t store_macroscopic) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  if (nsteps < 0) return fail(M3LB_EINVAL, "nsteps must be >= 0");
+  if (nsteps == 0) return M3LB_OK;
+  m3lb::StepArgs a = step_args(s);
+  if (store_macroscopic) {
+    for (int k = 0; k < 4; ++k) {
+      if (!s->macro_d[k]) HIP_OK(hipMalloc(&s->macro_d[k], size_t(s->d.np) * size_t(s->ks->real_bytes)));
+    }
+  }
+  const unsigned rows_per_block = unsigned(m3lb::kBlock >> a.bx_log2);
+  const unsigned grid = unsigned((a.rows + rows_per_block - 1) / rows_per_block);
+  for (int k = 0; k < nsteps; ++k) {
+    const bool last = k == nsteps - 1;
+    for (int j = 0; j < 4; ++j) a.macro[j] = (store_macroscopic && last) ? s->macro_d[j] : nullptr;
+    s->ks->step(a, int(s->t % 2), s->geometry, grid, s->stream);
+    ++s->t;
+  }
+  s->macro_valid = store_macroscopic != 0;
+  return launch_ok();
+}
+
+int m3lb_time_steps(m3lb_solver* s, int32_t nsteps, double* ms) {
+  if (!s || !ms) return fail(M3LB_EINVAL, "null argument");
+  HIP_OK(hipEventRecord(s->ev0, s->stream));
+  int rc = m3lb_step(s, nsteps, 0);
+  if (rc) return rc;
+  HIP_OK(hipEventRecord(s->ev1, s->stream));
+  HIP_OK(hipEventSynchronize(s->ev1));
+  float t = 0;
+  HIP_OK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+  *ms = double(t);
+  return M3LB_OK;
+}
+
+int m3lb_compute_macroscopic(m3lb_solver* s, double* rho, double* ux, double* uy, double* uz) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  const size_t np = size_t(s->d.np);
+  double* buf = nullptr;
+  HIP_OK(hipMalloc(&buf, 4 * np * sizeof(double)));
+  m3lb::ReadArgs a = read_args(s);
+  for (int k = 0; k < 4; ++k) a.out2[k] = buf + k * np;
+  s->ks->macro(a, s->stream);
+  int rc = launch_ok();
+  double* dst[4] = {rho, ux, uy, uz};
+  for (int k = 0; k < 4 && !rc; ++k)
+    if (dst[k] && hipMemcpyAsync(dst[k], buf + k * np, np * sizeof(double), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+      rc = fail(M3LB_EDEVICE, "macroscopic download failed");
+  hipError_t e = hipStreamSynchronize(s->stream);
+  hipFree(buf);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(M3LB_EDEVICE, hipGetErrorString(e));
+  return M3LB_OK;
+}
+
+int m3lb_macroscopic_stored(m3lb_solver* s, double* rho, double* ux, double* uy, double* uz) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  if (!s->macro_valid) return fail(M3LB_ESTATE, "no step with store_macroscopic = 1 since the last change");
+  const size_t np = size_t(s->d.np);
+  const size_t rb = size_t(s->ks->real_bytes);
+  std::vector<char> tmp(np * rb);
+  double* dst[4] = {rho, ux, uy, uz};
+  for (int k = 0; k < 4; ++k) {
+    if (!dst[k]) continue;
+    HIP_OK(hipMemcpyAsync(tmp.data(), s->macro_d[k], np * rb, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    for (size_t n = 0; n < np; ++n)
+      dst[k][n] = rb == 8 ? reinterpret_cast<const double*>(tmp.data())[n]
+                          : double(reinterpret_cast<const float*>(tmp.data())[n]);
+    // non-fluid nodes: the step never writes them; report 0 as compute_macroscopic does
+    for (size_t n = 0; n < np; ++n)
+      if (s->flags_h[n] != M3LB_FLUID) dst[k][n] = 0.0;
+  }
+  return M3LB_OK;
+}
+
+static int reduce4(m3lb_solver* s, double out[4]) {
+  const unsigned grid = reduce_grid(s);
+  int rc = ensure_partials(s, size_t(grid) * 4);
+  if (rc) return rc;
+  m3lb::ReadArgs a = read_args(s);
+  a.out = s->partial_d;
+  s->ks->reduce_mass(a, grid, s->stream);
+  rc = launch_ok();
+  if (rc) return rc;
+  std::vector<double> h(size_t(grid) * 4);
+  HIP_OK(hipMemcpyAsync(h.data(), s->partial_d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 4; ++k) out[k] = 0;
+  for (unsigned b = 0; b < grid; ++b)
+    for (int k = 0; k < 4; ++k) out[k] += h[size_t(b) * 4 + k];
+  return M3LB_OK;
+}
+
+int m3lb_total_mass(m3lb_solver* s, double* out) {
+  if (!s || !out) return fail(M3LB_EINVAL, "null argument");
+  double r[4];
+  int rc = reduce4(s, r);
+  if (rc) return rc;
+  *out = r[0];
+  return M3LB_OK;
+}
+
+int m3lb_total_momentum(m3lb_solver* s, int32_t axis, double* out) {
+  if (!s || !out) return fail(M3LB_EINVAL, "null argument");
+  if (axis < 0 || axis > 2) return fail(M3LB_EINVAL, "axis must be 0, 1 or 2");
+  double r[4];
+  int rc = reduce4(s, r);
+  if (rc) return rc;
+  *out = r[1 + axis];
+  return M3LB_OK;
+}
+
+int m3lb_storage_sum_exact(m3lb_solver* s, int64_t out[2]) {
+  if (!s || !out) return fail(M3LB_EINVAL, "null argument");
+  const unsigned grid = reduce_grid(s);
+  int rc = ensure_partials(s, size_t(grid) * 2);
+  if (rc) return rc;
+  m3lb::ReadArgs a = read_args(s);
+  a.out = s->partial_d;
+  s->ks->sum_exact(a, grid, s->stream);
+  rc = launch_ok();
+  if (rc) return rc;
+  std::vector<uint64_t> h(size_t(grid) * 2);
+  HIP_OK(hipMemcpyAsync(h.data(), s->partial_d, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  unsigned __int128 acc = 0;
+  for (unsigned b = 0; b < grid; ++b)
+    acc += ((unsigned __int128)h[2 * b] << 64) | (unsigned __int128)h[2 * b + 1];
+  out[0] = int64_t(uint64_t(acc >> 64));
+  out[1] = int64_t(uint64_t(acc));
+  return M3LB_OK;
+}
+
+}  // extern "C"
+
+// E = sum 1/2|u|^2 and Omega = sum 1/2|curl u|^2 over interior nodes with
+// periodic-wrap central differences (validation/Campaign.hpp:99-130).
+namespace {
+__global__ void __launch_bounds__(m3lb::kBlock) k_energy(m3lb::Dom d, const double* ux,
+                                                         const double* uy, const double* uz,
+                                                         double* out, int* finite) {
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  double acc[2] = {0, 0};
+  bool ok = true;
+  for (long long i = blockIdx.x * (long long)m3lb::kBlock + threadIdx.x; i < N;
+       i += (long long)gridDim.x * m3lb::kBlock) {
+    const int x = int(i % d.nx), y = int((i / d.nx) % d.ny), z = int(i / ((long long)d.nx * d.ny));
+    auto id = [&](int xx, int yy, int zz) {
+      xx = (xx + d.nx) % d.nx; yy = (yy + d.ny) % d.ny; zz = (zz + d.nz) % d.nz;
+      return ((zz + d.hz) * d.sy + (yy + d.hy)) * d.sx + xx + d.hx;
+    };
+    const int n = id(x, y, z);
+    const double a = ux[n], b = uy[n], c = uz[n];
+    if (!isfinite(a) || !isfinite(b) || !isfinite(c)) { ok = false; continue; }
+    acc[0] += 0.5 * (a * a + b * b + c * c);
+    const double wz = 0.5 * (uy[id(x + 1, y, z)] - uy[id(x - 1, y, z)]) -
+                      0.5 * (ux[id(x, y + 1, z)] - ux[id(x, y - 1, z)]);
+    double wx = 0, wy = 0;
+    if (d.nz > 1) {
+      wx = 0.5 * (uz[id(x, y + 1, z)] - uz[id(x, y - 1, z)]) -
+           0.5 * (uy[id(x, y, z + 1)] - uy[id(x, y, z - 1)]);
+      wy = 0.5 * (ux[id(x, y, z + 1)] - ux[id(x, y, z - 1)]) -
+           0.5 * (uz[id(x + 1, y, z)] - uz[id(x - 1, y, z)]);
+    }
+    acc[1] += 0.5 * (wx * wx + wy * wy + wz * wz);
+  }
+  if (!ok) *finite = 0;
+  m3lb::block_sum<2>(acc, out);
+}
+}  // namespace
+
+extern "C" int m3lb_energy_enstrophy(m3lb_solver* s, double* energy, double* enstrophy,
+                                     int32_t* finite) {
+  if (!s || !energy || !enstrophy) return fail(M3LB_EINVAL, "null argument");
+  const size_t np = size_t(s->d.np);
+  double* buf = nullptr;
+  int* fl = nullptr;
+  HIP_OK(hipMalloc(&buf, 3 * np * sizeof(double)));
+  HIP_OK(hipMalloc(&fl, sizeof(int)));
+  const int one = 1;
+  HIP_OK(hipMemcpyAsync(fl, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
+  m3lb::ReadArgs a = read_args(s);
+  a.out2[0] = nullptr;
+  for (int k = 0; k < 3; ++k) a.out2[1 + k] = buf + k * np;
+  s->ks->macro(a, s->stream);
+  const long long N = (long long)s->d.nx * s->d.ny * s->d.nz;
+  const unsigned grid = std::max(1u, std::min(4096u, unsigned((N + m3lb::kBlock - 1) / m3lb::kBlock)));
+  int rc = ensure_partials(s, size_t(grid) * 2);
+  if (!rc) {
+    k_energy<<<grid, m3lb::kBlock, 0, s->stream>>>(s->d, buf, buf + np, buf + 2 * np, s->partial_d, fl);
+    rc = launch_ok();
+  }
+  std::vector<double> h(size_t(grid) * 2);
+  int okv = 1;
+  if (!rc) {
+    if (hipMemcpyAsync(h.data(), s->partial_d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipMemcpyAsync(&okv, fl, sizeof(int), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+      rc = fail(M3LB_EDEVICE, "energy download failed");
+  }
+  hipError_t e = hipStreamSynchronize(s->stream);
+  hipFree(buf);
+  hipFree(fl);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(M3LB_EDEVICE, hipGetErrorString(e));
+  double E = 0, W = 0;
+  for (unsigned b = 0; b < grid; ++b) { E += h[2 * b]; W += h[2 * b + 1]; }
+  *energy = E;
+  *enstrophy = W;
+  if (finite) *finite = okv;
+  return M3LB_OK;
+}
+
+extern "C" {
+
+static int read_pops(m3lb_solver* s, double* out, bool logical) {
+  if (!s || !out) return fail(M3LB_EINVAL, "null argument");
+  const size_t n = size_t(s->d.np) * size_t(s->Q);
+  double* buf = nullptr;
+  HIP_OK(hipMalloc(&buf, n * sizeof(double)));
+  m3lb::ReadArgs a = read_args(s);
+  a.out = buf;
+  if (logical) s->ks->gather(a, s->stream);
+  else s->ks->storage(a, s->stream);
+  int rc = launch_ok();
+  if (!rc && hipMemcpyAsync(out, buf, n * sizeof(double), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+    rc = fail(M3LB_EDEVICE, "population download failed");
+  hipError_t e = hipStreamSynchronize(s->stream);
+  hipFree(buf);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(M3LB_EDEVICE, hipGetErrorString(e));
+  return M3LB_OK;
+}
+
+int m3lb_gather_populations(m3lb_solver* s, double* out) { return read_pops(s, out, true); }
+int m3lb_storage(m3lb_solver* s, double* out) { return read_pops(s, out, false); }
+
+int m3lb_timestep(const m3lb_solver* s, uint64_t* t) {
+  if (!s || !t) return fail(M3LB_EINVAL, "null argument");
+  *t = s->t;
+  return M3LB_OK;
+}
+
+int m3lb_set_force(m3lb_solver* s, double fx, double fy, double fz) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  if (s->cfg.forcing != M3LB_FORCE_GUO) return fail(M3LB_ESTATE, "the handle was created without Guo forcing");
+  s->cfg.force[0] = fx; s->cfg.force[1] = fy; s->cfg.force[2] = fz;
+  return M3LB_OK;
+}
+
+int m3lb_set_omega(m3lb_solver* s, double omega, double omega_minus, double omega_bulk) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  if (!(std::isfinite(omega) && std::isfinite(omega_minus) && std::isfinite(omega_bulk)))
+    return fail(M3LB_EINVAL, "relaxation rates must be finite");
+  s->cfg.omega = omega; s->cfg.omega_minus = omega_minus; s->cfg.omega_bulk = omega_bulk;
+  return M3LB_OK;
+}
+
+int m3lb_get_scales(m3lb_solver* s, double* sc) {
+  if (!s || !sc) return fail(M3LB_EINVAL, "null argument");
+  for (int q = 0; q < s->Q; ++q) sc[q] = s->scale[q];
+  return M3LB_OK;
+}
+
+static int apply_scales(m3lb_solver* s, const double* want) {
+  m3lb::Ratio r;
+  for (int q = 0; q < 27; ++q) r.r[q] = 1.0f;
+  for (int q = 0; q < s->Q; ++q) r.r[q] = float(want[q] / double(s->scale[q]));
+  const unsigned grid = reduce_grid(s);
+  s->ks->rescale(s->f, s->stride, s->d.np, r, grid, s->stream);
+  int rc = launch_ok();
+  if (rc) return rc;
+  for (int q = 0; q < s->Q; ++q) {
+    s->scale[q] = float(want[q]);
+    s->inv_scale[q] = float(1.0 / want[q]);
+  }
+  return sync(s);
+}
+
+int m3lb_set_scales(m3lb_solver* s, const double* sc) {
+  if (!s || !sc) return fail(M3LB_EINVAL, "null argument");
+  if (!is16(s)) return M3LB_OK;
+  for (int q = 0; q < s->Q; ++q) {
+    int e;
+    if (!(sc[q] > 0) || std::frexp(sc[q], &e) != 0.5)
+      return fail(M3LB_EINVAL, "scales must be positive powers of two");
+    const int o = q == 0 ? 0 : (q & 1 ? q + 1 : q - 1);
+    if (sc[q] != sc[o]) return fail(M3LB_EINVAL, "opposite directions must share a scale");
+  }
+  return apply_scales(s, sc);
+}
+
+int m3lb_device_bytes(const m3lb_solver* s, uint64_t* bytes) {
+  if (!s || !bytes) return fail(M3LB_EINVAL, "null argument");
+  uint64_t b = s->stride * uint64_t(s->Q) * uint64_t(s->ks->elem_bytes) + uint64_t(s->d.np);
+  for (void* p : s->macro_d)
+    if (p) b += uint64_t(s->d.np) * uint64_t(s->ks->real_bytes);
+  b += s->partial_n * sizeof(double);
+  *bytes = b;
+  return M3LB_OK;
+}
+
+int m3lb_calibrate_scales(m3lb_solver* s) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  return is16(s) ? calibrate(s) : M3LB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Largest power-of-two scale per opposite pair that keeps max |g_q| * s_q
+// below 2^14 (a factor 4 of headroom under FP16's largest finite 65504).
+int calibrate(m3lb_solver* s) {
+  const unsigned grid = reduce_grid(s);
+  int rc = ensure_partials(s, size_t(grid) * size_t(s->Q));
+  if (rc) return rc;
+  m3lb::ReadArgs a = read_args(s);
+  a.out = s->partial_d;
+  s->ks->absmax(a, grid, s->stream);
+  rc = launch_ok();
+  if (rc) return rc;
+  std::vector<double> h(size_t(grid) * size_t(s->Q));
+  HIP_OK(hipMemcpyAsync(h.data(), s->partial_d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  std::vector<double> mx(size_t(s->Q), 0.0), want(size_t(s->Q));
+  for (unsigned b = 0; b < grid; ++b)
+    for (int q = 0; q < s->Q; ++q) mx[size_t(q)] = std::max(mx[size_t(q)], h[size_t(b) * s->Q + q]);
+  for (int q = 0; q < s->Q; ++q) {
+    const int o = q == 0 ? 0 : (q & 1 ? q + 1 : q - 1);
+    const double m = std::max(mx[size_t(q)], mx[size_t(o)]);
+    int k = m > 0 ? int(std::floor(std::log2(16384.0 / m))) : 0;
+    k = std::max(-60, std::min(60, k));
+    want[size_t(q)] = std::ldexp(1.0, k);
+  }
+  return apply_scales(s, want.data());
+}
+}  // namespace

@@ -1,0 +1,218 @@
+"""Python mirror of FluidSolver<L, EsotericPull<L>, Collision> over libm3lb_hip.so.
+
+Method names follow the reference's public solver API
+(src/solver/FluidSolver.hpp:42-637); every call goes through the C-ABI in
+include/m3lb.h.  There is no CPU fallback: if the library is missing or a call
+fails, a RuntimeError carries the library's message.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libm3lb_hip.so")
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_cfgp = ctypes.POINTER(abi.Config)
+_vp = ctypes.c_void_p
+
+# Every entry point of include/m3lb.h with its ctypes signature.
+SIGNATURES = {
+    "m3lb_domain": (ctypes.c_int, [_cfgp, _i64p]),
+    "m3lb_create": (ctypes.c_int, [_cfgp, ctypes.POINTER(_vp)]),
+    "m3lb_destroy": (ctypes.c_int, [_vp]),
+    "m3lb_last_error": (ctypes.c_char_p, []),
+    "m3lb_set_geometry": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8)]),
+    "m3lb_initialize": (ctypes.c_int, [_vp] + [ctypes.c_double] * 4),
+    "m3lb_initialize_field": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "m3lb_step": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    "m3lb_compute_macroscopic": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "m3lb_macroscopic_stored": (ctypes.c_int, [_vp, _dp, _dp, _dp, _dp]),
+    "m3lb_total_mass": (ctypes.c_int, [_vp, _dp]),
+    "m3lb_total_momentum": (ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    "m3lb_storage_sum_exact": (ctypes.c_int, [_vp, _i64p]),
+    "m3lb_energy_enstrophy": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int32)]),
+    "m3lb_gather_populations": (ctypes.c_int, [_vp, _dp]),
+    "m3lb_seed_populations": (ctypes.c_int, [_vp, _dp]),
+    "m3lb_storage": (ctypes.c_int, [_vp, _dp]),
+    "m3lb_timestep": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "m3lb_set_force": (ctypes.c_int, [_vp] + [ctypes.c_double] * 3),
+    "m3lb_set_omega": (ctypes.c_int, [_vp] + [ctypes.c_double] * 3),
+    "m3lb_get_scales": (ctypes.c_int, [_vp, _dp]),
+    "m3lb_set_scales": (ctypes.c_int, [_vp, _dp]),
+    "m3lb_calibrate_scales": (ctypes.c_int, [_vp]),
+    "m3lb_time_steps": (ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
+    "m3lb_device_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+}
+
+
+def load():
+    """Loads libm3lb_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                           "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _LIB.m3lb_last_error().decode()
+        raise RuntimeError(f"m3lb error {rc}: {msg}")
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def domain(cfg):
+    """m3lb_domain: (nx, ny, nz, hx, hy, hz, sx, sy, sz, n_padded)."""
+    L = load()
+    out = (ctypes.c_int64 * 10)()
+    _check(L.m3lb_domain(ctypes.byref(cfg), out))
+    return tuple(out)
+
+
+class FluidSolver:
+    """One device-resident lattice (FluidSolver.hpp:34-69)."""
+
+    def __init__(self, cfg):
+        self.L = load()
+        self.cfg = cfg
+        self.Q = 9 if cfg.lattice == abi.D2Q9 else 27
+        self.dom = domain(cfg)
+        self.np = self.dom[9]
+        h = _vp()
+        _check(self.L.m3lb_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.m3lb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- setup ---------------------------------------------------------------
+    def set_geometry(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.uint8).reshape(-1)
+        assert f.size == self.cfg.nx * self.cfg.ny * self.cfg.nz
+        _check(self.L.m3lb_set_geometry(self.h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+
+    def initialize(self, rho0, ux=0.0, uy=0.0, uz=0.0):
+        _check(self.L.m3lb_initialize(self.h, rho0, ux, uy, uz))
+
+    def initialize_field(self, rho, ux, uy, uz):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rho, ux, uy, uz)]
+        for a in arrs:
+            assert a.size == self.np
+        _check(self.L.m3lb_initialize_field(self.h, *[_d(a) for a in arrs]))
+
+    def seed_populations(self, pops):
+        a = np.ascontiguousarray(pops, dtype=np.float64).reshape(-1)
+        assert a.size == self.np * self.Q
+        _check(self.L.m3lb_seed_populations(self.h, _d(a)))
+
+    # -- stepping ------------------------------------------------------------
+    def step(self, n=1, store_macroscopic=False):
+        _check(self.L.m3lb_step(self.h, int(n), 1 if store_macroscopic else 0))
+
+    def time_steps(self, n):
+        ms = ctypes.c_double()
+        _check(self.L.m3lb_time_steps(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    # -- readout -------------------------------------------------------------
+    def compute_macroscopic(self):
+        out = [np.zeros(self.np) for _ in range(4)]
+        _check(self.L.m3lb_compute_macroscopic(self.h, *[_d(a) for a in out]))
+        return out
+
+    def macroscopic_stored(self):
+        out = [np.zeros(self.np) for _ in range(4)]
+        _check(self.L.m3lb_macroscopic_stored(self.h, *[_d(a) for a in out]))
+        return out
+
+    def total_mass(self):
+        v = ctypes.c_double()
+        _check(self.L.m3lb_total_mass(self.h, ctypes.byref(v)))
+        return v.value
+
+    def total_momentum(self, axis):
+        v = ctypes.c_double()
+        _check(self.L.m3lb_total_momentum(self.h, int(axis), ctypes.byref(v)))
+        return v.value
+
+    def storage_sum_exact(self):
+        o = (ctypes.c_int64 * 2)()
+        _check(self.L.m3lb_storage_sum_exact(self.h, o))
+        return (o[0], o[1])
+
+    def energy_enstrophy(self):
+        e, w, ok = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+        _check(self.L.m3lb_energy_enstrophy(self.h, ctypes.byref(e), ctypes.byref(w),
+                                            ctypes.byref(ok)))
+        return e.value, w.value, bool(ok.value)
+
+    def gather_populations(self):
+        out = np.zeros(self.np * self.Q)
+        _check(self.L.m3lb_gather_populations(self.h, _d(out)))
+        return out.reshape(self.np, self.Q)
+
+    def storage(self):
+        out = np.zeros(self.np * self.Q)
+        _check(self.L.m3lb_storage(self.h, _d(out)))
+        return out.reshape(self.np, self.Q)
+
+    def timestep(self):
+        t = ctypes.c_uint64()
+        _check(self.L.m3lb_timestep(self.h, ctypes.byref(t)))
+        return t.value
+
+    def set_force(self, fx, fy, fz):
+        _check(self.L.m3lb_set_force(self.h, fx, fy, fz))
+
+    def set_omega(self, omega, omega_minus=None, omega_bulk=-1.0):
+        if omega_minus is None:
+            omega_minus = omega
+        _check(self.L.m3lb_set_omega(self.h, omega, omega_minus, omega_bulk))
+
+    def scales(self):
+        out = np.zeros(self.Q)
+        _check(self.L.m3lb_get_scales(self.h, _d(out)))
+        return out
+
+    def set_scales(self, s):
+        a = np.ascontiguousarray(s, dtype=np.float64)
+        _check(self.L.m3lb_set_scales(self.h, _d(a)))
+
+    def calibrate_scales(self):
+        _check(self.L.m3lb_calibrate_scales(self.h))
+
+    def device_bytes(self):
+        b = ctypes.c_uint64()
+        _check(self.L.m3lb_device_bytes(self.h, ctypes.byref(b)))
+        return b.value

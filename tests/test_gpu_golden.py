@@ -1,0 +1,100 @@
+"""The HIP path against the reference's own committed results (FP64, the
+precision they were produced in), through the C-ABI.  Same drivers as
+tests/test_oracle_golden.py; here every ladder runs to its largest size."""
+import math
+import sys
+
+import pytest
+
+from golden_cases import (GAL, GOLD, OPS, PRINT_TOL, TG2, decaying, poiseuille, rel,
+                          tgv2d_run, tgv3d_run)
+from m3lb_amd import abi
+from m3lb_amd.solver import FluidSolver
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def announce(request):
+    print(f"\n[gpu-case] {request.node.name}", flush=True)
+    sys.stdout.flush()
+    yield
+
+
+@pytest.mark.parametrize("op", list(OPS))
+def test_taylor_green_ladder(op):
+    """doc/fig/conv_taylor_green_<op>.dat, on D2Q9 and (Nz = 8) D3Q27."""
+    for N, err in GOLD["grid_convergence"]["taylor_green"][op]["rows"]:
+        for lat in (abi.D2Q9, abi.D3Q27):
+            got, _, _ = decaying("taylor_green", lat, OPS[op], int(N), solver_cls=FluidSolver)
+            assert rel(got, err) < PRINT_TOL, (op, lat, N, got, err)
+
+
+@pytest.mark.parametrize("op", list(OPS))
+def test_abc_ladder(op):
+    """doc/fig/conv_abc_<op>.dat, N = 16 ... 48."""
+    for N, err in GOLD["grid_convergence"]["abc"][op]["rows"]:
+        got, _, _ = decaying("abc", abi.D3Q27, OPS[op], int(N), solver_cls=FluidSolver)
+        assert rel(got, err) < PRINT_TOL, (op, N, got, err)
+
+
+@pytest.mark.parametrize("op", list(OPS))
+def test_galilean_nu_eff(op):
+    g = GOLD["galilean_nu_eff"]
+    for U0, want in zip(g["U0"], g["rows"][GAL[op]]):
+        _, nu_eff, _ = decaying("abc", abi.D3Q27, OPS[op], 32, U=0.01, U0=U0,
+                                solver_cls=FluidSolver)
+        assert abs(nu_eff - want) <= 5.01e-7, (op, U0, nu_eff, want)
+
+
+@pytest.mark.parametrize("name", list(TG2))
+def test_tgv2d_results(name):
+    """results/A_taylor_green/tgv2d_<name>.dat, every committed N <= 128."""
+    lat, op = TG2[name]
+    for N, _tau, steps, err in GOLD["tgv2d"][name]["rows"]:
+        if N > 128:
+            continue
+        got, T = tgv2d_run(lat, op, int(N), solver_cls=FluidSolver)
+        assert T == int(steps)
+        assert rel(got, err) < PRINT_TOL, (name, N, got, err)
+
+
+@pytest.mark.parametrize("col", ["BGK", "TRT", "MRT", "CM"])
+def test_poiseuille_slip_table(col):
+    g = GOLD["poiseuille_slip"]
+    ci = g["columns"].index(col)
+    for row in g["rows"]:
+        tau, want = row[0], row[ci]
+        if abs(tau - 0.9330127) < 1e-7:
+            tau = 0.5 + math.sqrt(3.0) / 4.0
+        _, rho_mean, slip, _ = poiseuille(OPS[col], tau, solver_cls=FluidSolver)
+        if abs(want) < 1e-11:
+            assert abs(slip) < 1e-11, (col, tau, slip)
+        else:
+            assert abs(slip - want) <= 0.0051 * abs(want), (col, tau, slip, want)
+        assert abs(rho_mean - 1.0) < 1e-9
+
+
+def test_poiseuille_exact_at_magic_points():
+    """validation/poiseuille.cpp:247-301: each operator exact (L2 < 1e-11) at
+    its magic point for H = 16, 32, 64; off-magic BGK second order."""
+    tb = 0.5 + math.sqrt(3.0) / 4.0
+    for op, tau in ((abi.OP_BGK, tb), (abi.OP_TRT, 0.6), (abi.OP_MRT, 0.875), (abi.OP_CM, 0.875)):
+        for H in (16, 32, 64):
+            l2, _, _, _ = poiseuille(op, tau, H=H, solver_cls=FluidSolver)
+            assert l2 < 1e-11, (op, tau, H, l2)
+    e = [poiseuille(abi.OP_BGK, 0.6, H=H, solver_cls=FluidSolver)[0] for H in (16, 32, 64)]
+    assert abs(math.log(e[1] / e[2]) / math.log(2.0) - 2.0) < 0.25
+
+
+def test_tgv3d_re1600_file():
+    """results/E_tgv3d/tgv3d_re1600_d3q27_cm.dat: all 20 probes (32000 steps at
+    64^3, FP64), E/E0 and enstrophy ratio."""
+    rows = GOLD["tgv3d_re1600_d3q27_cm"]["rows"]
+    got = tgv3d_run(FluidSolver, len(rows) - 1)
+    worst = 0.0
+    for (t, E, W), (tg, Eg, Wg) in zip(got, rows):
+        assert abs(t - tg) < 1e-9
+        worst = max(worst, rel(E, Eg), rel(W, Wg))
+    print(f"tgv3d worst relative difference {worst:.3e}")
+    assert worst < 1e-6

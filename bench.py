@@ -81,13 +81,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
-    ap.add_argument("--cpu-size", type=int, default=96)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-size", type=int, default=192)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true",
                     help="also time the 16-bit and FP64 storage forms and BGK (stderr)")
+    ap.add_argument("--stream-probe-only", type=int, default=0, metavar="BYTES",
+                    help="only run the plain in-place stream of BYTES (counter calibration)")
     args = ap.parse_args()
+
+    if args.stream_probe_only:
+        from m3lb_amd import solver
+        ms = solver.stream_probe(args.stream_probe_only, nontemporal=False, reps=3)
+        log(f"[bench] stream probe {args.stream_probe_only} B in place: {ms:.3f} ms, "
+            f"{2 * args.stream_probe_only / ms / 1e6:.0f} GB/s")
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -120,6 +129,28 @@ def main():
         log(f"[bench] cpu {args.cpu_size}^3: {mlups_cpu:.2f} MLUPS on {threads} threads")
 
     bpn = bytes_per_node(abi, dt)
+    # The device's streaming ceiling, measured now on an array the size of the
+    # population array: the best a no-reuse in-place kernel reaches here.
+    from m3lb_amd import solver
+    arr = bpn // 2 * args.size ** 3
+    ceil = {}
+    for nt in (False, True):
+        ms = solver.stream_probe(arr, nontemporal=nt, reps=3)
+        ceil["nontemporal" if nt else "plain"] = round(2 * arr / ms / 1e6, 1)
+    best_ceiling = max(ceil.values())
+    log(f"[bench] in-place stream of {arr / 1e9:.2f} GB: plain {ceil['plain']:.0f} GB/s, "
+        f"nontemporal {ceil['nontemporal']:.0f} GB/s; step kernel at "
+        f"{r['achieved_gbs'] / best_ceiling:.3f} of the better")
+
+    # HBM bytes per launch from the committed rocprofv3 --pmc summary of this
+    # exact workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel).
+    traffic, traffic_src = None, None
+    tj = os.path.join(ROOT, "profiles", "k_step_cm_f32_512cubed_traffic.json")
+    if args.size == 512 and args.dtype == "f32" and os.path.exists(tj):
+        with open(tj) as fh:
+            traffic = json.load(fh)["traffic_bytes"]
+        traffic_src = "profiles/k_step_cm_f32_512cubed.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+
     out = {
         "metric": "D3Q27 CM MLUPS + %HBM roofline at 512^3",
         "value": round(r["mlups"], 1),
@@ -140,8 +171,11 @@ def main():
                    "nodes": args.size ** 3, "bytes_per_node_step": bpn},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None},
+                     "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "traffic_source": traffic_src,
+        "stream_ceiling_gbs": ceil,
+        "frac_of_stream_ceiling": round(r["achieved_gbs"] / best_ceiling, 4),
         "mass_drift": r["mass_drift"],
     }
     print(json.dumps(out), flush=True)

@@ -191,6 +191,28 @@ int m3lb_calibrate_scales(m3lb_solver* s);
  * counterpart in the reference. */
 int m3lb_time_steps(m3lb_solver* s, int32_t nsteps, double* ms);
 
+/* Order in which the step kernel's tiles are assigned to the GPU's 8 XCDs:
+ * 0 = hardware order, -1 = each XCD takes one contiguous eighth of the box,
+ * run > 0 = XCDs take turns with runs of `run` consecutive tiles (default 64
+ * for 4- and 8-byte populations, 2 for 16-bit storage).
+ * A tuning knob only: results are bit-identical for every value.  No
+ * counterpart in the reference. */
+int m3lb_set_tile_order(m3lb_solver* s, int32_t run);
+
+/* The block -> tile map the step kernel uses for `grid` blocks and a tile
+ * order `run` (host-side copy of the device function): out[b] = tile of block
+ * b.  A permutation of [0, grid) for every value; exposed so that this can be
+ * checked without a GPU. */
+int m3lb_tile_order_map(uint32_t grid, int32_t run, uint32_t* out);
+
+/* Streams `bytes` of device memory in place (read + write, one float4 per
+ * thread, nontemporal or not), `reps` times, and returns the fastest time in
+ * milliseconds: the streaming ceiling of the device, which the step kernel is
+ * measured against, and a copy kernel of known size for calibrating HBM
+ * traffic counters.  Allocates and frees its own buffer.  No counterpart in
+ * the reference. */
+int m3lb_stream_probe(uint64_t bytes, int32_t nontemporal, int32_t reps, double* best_ms);
+
 /* Device bytes held by the handle (populations + flags + macroscopic). */
 int m3lb_device_bytes(const m3lb_solver* s, uint64_t* bytes);
 

@@ -154,6 +154,67 @@ __device__ __forceinline__ typename K::R density(const Macro<typename K::R>& m) 
   else return m.dens;
 }
 
+// Raw monomial moments S_pqr = sum_i c_ix^p c_iy^q c_iz^r f_i of order <= 2,
+// from one factorised pass: sums and differences along z, then y, then x
+// (the first-order part of ProductBasis::to_moments at u_b = 0,
+// ProductBasis.hpp:295-317).  Unused moments are dead code after inlining.
+template <class R>
+struct RawMoments {
+  R S0, Sx, Sy, Sz, Sxx, Syy, Szz, Sxy, Sxz, Syz;
+};
+
+template <class K>
+__device__ __forceinline__ RawMoments<typename K::R> raw_moments(const typename K::R* f) {
+  using R = typename K::R;
+  constexpr int Q = K::Q;
+  RawMoments<R> S;
+  if constexpr (K::D == 3) {
+    R z0[3][3], z1[3][3], z2[3][3];
+    sfor<9>([&](auto AB) {
+      constexpr int a = AB / 3, b = AB % 3;
+      const R vm = f[dir_of<Q>(a, b, 0)], v0 = f[dir_of<Q>(a, b, 1)], vp = f[dir_of<Q>(a, b, 2)];
+      z2[a][b] = vp + vm;
+      z1[a][b] = vp - vm;
+      z0[a][b] = z2[a][b] + v0;
+    });
+    R Y00[3], Y10[3], Y20[3], Y01[3], Y11[3], Y02[3];
+    for (int a = 0; a < 3; ++a) {
+      Y20[a] = z0[a][2] + z0[a][0];
+      Y10[a] = z0[a][2] - z0[a][0];
+      Y00[a] = Y20[a] + z0[a][1];
+      Y11[a] = z1[a][2] - z1[a][0];
+      Y01[a] = (z1[a][2] + z1[a][0]) + z1[a][1];
+      Y02[a] = (z2[a][2] + z2[a][0]) + z2[a][1];
+    }
+    S.Sxx = Y00[2] + Y00[0]; S.Sx = Y00[2] - Y00[0]; S.S0 = S.Sxx + Y00[1];
+    S.Sxy = Y10[2] - Y10[0]; S.Sy = (Y10[2] + Y10[0]) + Y10[1];
+    S.Syy = (Y20[2] + Y20[0]) + Y20[1];
+    S.Sxz = Y01[2] - Y01[0]; S.Sz = (Y01[2] + Y01[0]) + Y01[1];
+    S.Syz = (Y11[2] + Y11[0]) + Y11[1];
+    S.Szz = (Y02[2] + Y02[0]) + Y02[1];
+  } else {
+    R Y0[3], Y1[3], Y2[3];
+    sfor<3>([&](auto A) {
+      constexpr int a = A;
+      const R vm = f[dir_of<Q>(a, 0, 0)], v0 = f[dir_of<Q>(a, 1, 0)], vp = f[dir_of<Q>(a, 2, 0)];
+      Y2[a] = vp + vm;
+      Y1[a] = vp - vm;
+      Y0[a] = Y2[a] + v0;
+    });
+    S.Sxx = Y0[2] + Y0[0]; S.Sx = Y0[2] - Y0[0]; S.S0 = S.Sxx + Y0[1];
+    S.Sxy = Y1[2] - Y1[0]; S.Sy = (Y1[2] + Y1[0]) + Y1[1];
+    S.Syy = (Y2[2] + Y2[0]) + Y2[1];
+    S.Sz = R(0); S.Szz = R(0); S.Sxz = R(0); S.Syz = R(0);
+  }
+  return S;
+}
+
+// rho and u from S0 and S_a (BGK.hpp:201-216) with Guo::shift_velocity
+// (Forcing.hpp:65-69).
+template <class K>
+__device__ __forceinline__ Macro<typename K::R> macro_of(const RawMoments<typename K::R>& S,
+                                                         const Params<typename K::R>& p);
+
 // macroscopic(): sum f, sum c f, u = m / rho, Guo half shift
 // (BGK.hpp:201-216, Forcing.hpp:65-69); NullCollision gives {1,0,0,0}.
 template <class K>
@@ -166,27 +227,24 @@ __device__ __forceinline__ Macro<typename K::R> macroscopic(const typename K::R*
     m.dens = R(1); m.ux = R(0); m.uy = R(0); m.uz = R(0);
     return m;
   } else {
-    R s = R(0), mx = R(0), my = R(0), mz = R(0);
-    sfor<Q>([&](auto I) {
-      constexpr int i = I;
-      s += f[i];
-      if constexpr (Lat<Q>::cx[i] > 0) mx += f[i];
-      if constexpr (Lat<Q>::cx[i] < 0) mx -= f[i];
-      if constexpr (Lat<Q>::cy[i] > 0) my += f[i];
-      if constexpr (Lat<Q>::cy[i] < 0) my -= f[i];
-      if constexpr (Lat<Q>::cz[i] > 0) mz += f[i];
-      if constexpr (Lat<Q>::cz[i] < 0) mz -= f[i];
-    });
-    m.dens = s;
-    const R rho = density<K>(m);
-    const R ir = R(1) / rho;
-    m.ux = mx * ir; m.uy = my * ir; m.uz = mz * ir;
-    if constexpr (K::FORCE) {
-      const R h = R(0.5) * ir;
-      m.ux += h * p.F[0]; m.uy += h * p.F[1]; m.uz += h * p.F[2];
-    }
-    return m;
+    return macro_of<K>(raw_moments<K>(f), p);
   }
+}
+
+template <class K>
+__device__ __forceinline__ Macro<typename K::R> macro_of(const RawMoments<typename K::R>& S,
+                                                         const Params<typename K::R>& p) {
+  using R = typename K::R;
+  Macro<R> m;
+  m.dens = S.S0;
+  const R rho = density<K>(m);
+  const R ir = R(1) / rho;
+  m.ux = S.Sx * ir; m.uy = S.Sy * ir; m.uz = S.Sz * ir;
+  if constexpr (K::FORCE) {
+    const R h = R(0.5) * ir;
+    m.ux += h * p.F[0]; m.uy += h * p.F[1]; m.uz += h * p.F[2];
+  }
+  return m;
 }
 
 // ---------------------------------------------------------------------------
@@ -467,6 +525,154 @@ __device__ __forceinline__ void collide_moment(typename K::R* f, const Macro<typ
       });
     });
   });
+}
+
+// The step's moment operators, fused with macroscopic(): the ten raw monomial
+// moments S_pqr = sum c_x^p c_y^q c_z^r f (p+q+r <= 2) come out of one
+// factorised pass (sums and differences along z, then y, then x), give rho and
+// u, and are shifted to the Hermite basis {1, C, C^2 - cs2} at u_b.  Exactly
+// the first ten moments of ProductBasis::to_moments (ProductBasis.hpp:295-317);
+// the moments of order >= 3 it would also produce are overwritten by their
+// equilibria (MomentCollision.hpp:287-288), so they are never formed.
+template <class K, bool CENTRAL>
+__device__ __forceinline__ Macro<typename K::R> collide_moment_fused(typename K::R* f,
+                                                                      const Params<typename K::R>& p) {
+  using R = typename K::R;
+  constexpr int Q = K::Q, D = K::D;
+  constexpr R cs2 = R(1) / R(3);
+  const RawMoments<R> RM = raw_moments<K>(f);
+  const Macro<R> m = macro_of<K>(RM, p);
+  const R rho = density<K>(m);
+  const R S0 = RM.S0, Sx = RM.Sx, Sy = RM.Sy, Sz = RM.Sz, Sxx = RM.Sxx, Syy = RM.Syy,
+          Szz = RM.Szz, Sxy = RM.Sxy, Sxz = RM.Sxz, Syz = RM.Syz;
+  const R u[3] = {m.ux, m.uy, m.uz};
+  R ub[3], du[3], u2m[3];
+  for (int a = 0; a < 3; ++a) {
+    ub[a] = CENTRAL ? u[a] : R(0);
+    du[a] = u[a] - ub[a];
+    u2m[a] = ub[a] * ub[a] - cs2;
+  }
+  R Qf[3][3], Aw[3][3];
+  for (int a = 0; a < 3; ++a) {
+    Qf[a][0] = R(1); Qf[a][1] = du[a]; Qf[a][2] = du[a] * du[a];
+    Aw[a][0] = R(1); Aw[a][1] = -ub[a]; Aw[a][2] = ub[a] * ub[a];
+  }
+  auto eqm = [&](auto N) -> R {
+    constexpr int n = N;
+    constexpr int pp = D == 2 ? n / 3 : n / 9;
+    constexpr int qq = D == 2 ? n % 3 : (n / 3) % 3;
+    constexpr int rr = D == 2 ? 0 : n % 3;
+    R e = rho * Qf[0][pp] * Qf[1][qq];
+    R w = Aw[0][pp] * Aw[1][qq];
+    if constexpr (D == 3) { e *= Qf[2][rr]; w *= Aw[2][rr]; }
+    if constexpr (K::SHIFTED) return e - w;
+    else return e;
+  };
+  // basis moments of order <= 2 at u_b
+  R k[27];
+  k[mslot<D>(0, 0, 0)] = S0;
+  k[mslot<D>(1, 0, 0)] = Sx - ub[0] * S0;
+  k[mslot<D>(0, 1, 0)] = Sy - ub[1] * S0;
+  k[mslot<D>(2, 0, 0)] = Sxx - R(2) * ub[0] * Sx + u2m[0] * S0;
+  k[mslot<D>(0, 2, 0)] = Syy - R(2) * ub[1] * Sy + u2m[1] * S0;
+  k[mslot<D>(1, 1, 0)] = Sxy - ub[0] * Sy - ub[1] * Sx + ub[0] * ub[1] * S0;
+  if constexpr (D == 3) {
+    k[mslot<D>(0, 0, 1)] = Sz - ub[2] * S0;
+    k[mslot<D>(0, 0, 2)] = Szz - R(2) * ub[2] * Sz + u2m[2] * S0;
+    k[mslot<D>(1, 0, 1)] = Sxz - ub[0] * Sz - ub[2] * Sx + ub[0] * ub[2] * S0;
+    k[mslot<D>(0, 1, 1)] = Syz - ub[1] * Sz - ub[2] * Sy + ub[1] * ub[2] * S0;
+  }
+
+  const R omega = p.omega;
+  const R wb = p.omega_b < R(0) ? omega : p.omega_b;
+  const R w2 = R(1) - R(0.5) * omega;
+  if constexpr (K::FORCE) {
+    k[mslot<D>(1, 0, 0)] += p.F[0];
+    k[mslot<D>(0, 1, 0)] += p.F[1];
+    if constexpr (D == 3) k[mslot<D>(0, 0, 1)] += p.F[2];
+  }
+  {
+    constexpr int i2d[3] = {mslot<D>(2, 0, 0), mslot<D>(0, 2, 0), mslot<D>(0, 0, 2)};
+    R d[3], e[3], g[3];
+    R tr = R(0), tre = R(0), trg = R(0);
+    sfor<D>([&](auto A) {
+      constexpr int a = A;
+      d[a] = k[i2d[a]];
+      e[a] = eqm(std::integral_constant<int, i2d[a]>{});
+      g[a] = K::FORCE ? R(2) * p.F[a] * du[a] : R(0);
+      tr += d[a]; tre += e[a]; trg += g[a];
+    });
+    constexpr R invD = R(1) / R(D);
+    const R tr_post = (R(1) - wb) * tr + wb * tre + (R(1) - R(0.5) * wb) * trg;
+    sfor<D>([&](auto A) {
+      constexpr int a = A;
+      k[i2d[a]] = (R(1) - omega) * (d[a] - tr * invD) + omega * (e[a] - tre * invD) +
+                  w2 * (g[a] - trg * invD) + tr_post * invD;
+    });
+    sfor<D>([&](auto A) {
+      sfor<D>([&](auto B) {
+        constexpr int a = A, b = B;
+        if constexpr (b > a) {
+          constexpr int id = mslot<D>((a == 0 || b == 0), (a == 1 || b == 1),
+                                      D == 3 && (a == 2 || b == 2));
+          const R ge = K::FORCE ? (p.F[b] * du[a] + p.F[a] * du[b]) : R(0);
+          k[id] = (R(1) - omega) * k[id] + omega * eqm(std::integral_constant<int, id>{}) +
+                  w2 * ge;
+        }
+      });
+    });
+  }
+  sfor<(D == 3 ? 27 : 9)>([&](auto N) {
+    constexpr int n = N;
+    constexpr int ord = D == 2 ? (n / 3 + n % 3) : (n / 9 + (n / 3) % 3 + n % 3);
+    if constexpr (ord >= 3) k[n] = eqm(N);
+  });
+  if constexpr (D == 3) {
+    sfor<9>([&](auto QR) {
+      constexpr int q = QR / 3, r = QR % 3;
+      inv1d(k[mslot<3>(0, q, r)], k[mslot<3>(1, q, r)], k[mslot<3>(2, q, r)], ub[0], u2m[0]);
+    });
+    sfor<9>([&](auto AR) {
+      constexpr int a = AR / 3, r = AR % 3;
+      inv1d(k[mslot<3>(a, 0, r)], k[mslot<3>(a, 1, r)], k[mslot<3>(a, 2, r)], ub[1], u2m[1]);
+    });
+    sfor<9>([&](auto AB) {
+      constexpr int a = AB / 3, b = AB % 3;
+      inv1d(k[mslot<3>(a, b, 0)], k[mslot<3>(a, b, 1)], k[mslot<3>(a, b, 2)], ub[2], u2m[2]);
+    });
+  } else {
+    sfor<3>([&](auto Qd) {
+      constexpr int q = Qd;
+      inv1d(k[q], k[3 + q], k[6 + q], ub[0], u2m[0]);
+    });
+    sfor<3>([&](auto A) {
+      constexpr int a = A;
+      inv1d(k[a * 3 + 0], k[a * 3 + 1], k[a * 3 + 2], ub[1], u2m[1]);
+    });
+  }
+  sfor<3>([&](auto A) {
+    sfor<3>([&](auto B) {
+      sfor<(D == 3 ? 3 : 1)>([&](auto C) {
+        f[dir_of<Q>(A, B, C)] = k[mslot<D>(A, B, C)];
+      });
+    });
+  });
+  return m;
+}
+
+// macroscopic() + collide() as the step kernel runs them.
+template <class K>
+__device__ __forceinline__ Macro<typename K::R> collide_fused(typename K::R* f,
+                                                             const Params<typename K::R>& p) {
+  if constexpr (K::OP == M3LB_OP_MRT) {
+    return collide_moment_fused<K, false>(f, p);
+  } else if constexpr (K::OP == M3LB_OP_CM) {
+    return collide_moment_fused<K, true>(f, p);
+  } else {
+    const Macro<typename K::R> m = macroscopic<K>(f, p);
+    collide<K>(f, m, p);
+    return m;
+  }
 }
 
 template <class K>

@@ -99,6 +99,7 @@ struct m3lb_solver {
   float scale[27];
   float inv_scale[27];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int remap = 64;  // step-kernel tile order (kernels.hpp tile_of)
 };
 
 namespace {
@@ -125,7 +126,9 @@ m3lb::StepArgs step_args(const m3lb_solver* s) {
   a.d = s->d;
   const int nx = s->d.nx;
   a.bx_log2 = nx > 128 ? 8 : (nx > 64 ? 7 : 6);
+  a.chunks = (nx + m3lb::kBlock - 1) / m3lb::kBlock;
   a.rows = s->d.ny * s->d.nz;
+  a.remap = s->remap;
   a.omega = s->cfg.omega;
   a.omega_m = s->cfg.omega_minus;
   a.omega_b = s->cfg.omega_bulk;
@@ -237,6 +240,9 @@ int m3lb_create(const m3lb_config* cfg, m3lb_solver** out) {
   s->ks = ks;
   s->stride = (uint64_t(np) + 63) / 64 * 64;
   provisional_scales(s);
+  // Tile order measured fastest at 512^3 on MI355X: runs of 64 tiles per XCD
+  // for 4- and 8-byte populations, runs of 2 (one row) for 2-byte ones.
+  s->remap = ks->elem_bytes == 2 ? 2 : 64;
 
   auto bail = [&](int code) {
     m3lb_destroy(s);
@@ -371,7 +377,8 @@ int m3lb_step(m3lb_solver* s, int32_t nsteps, int32_t store_macroscopic) {
     }
   }
   const unsigned rows_per_block = unsigned(m3lb::kBlock >> a.bx_log2);
-  const unsigned grid = unsigned((a.rows + rows_per_block - 1) / rows_per_block);
+  const unsigned grid = a.chunks > 1 ? unsigned(a.rows) * unsigned(a.chunks)
+                                     : unsigned((a.rows + rows_per_block - 1) / rows_per_block);
   for (int k = 0; k < nsteps; ++k) {
     const bool last = k == nsteps - 1;
     for (int j = 0; j < 4; ++j) a.macro[j] = (store_macroscopic && last) ? s->macro_d[j] : nullptr;
@@ -530,7 +537,57 @@ __global__ void __launch_bounds__(m3lb::kBlock) k_energy(m3lb::Dom d, const doub
   if (!ok) *finite = 0;
   m3lb::block_sum<2>(acc, out);
 }
+
+// Plain in-place stream, one float4 per thread: the best a memory-bound
+// kernel with no reuse reaches on this device.
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <bool NT>
+__global__ void __launch_bounds__(m3lb::kBlock) k_stream(f4v* a, uint64_t n4) {
+  const uint64_t i = uint64_t(blockIdx.x) * m3lb::kBlock + threadIdx.x;
+  if (i >= n4) return;
+  f4v v;
+  if constexpr (NT) v = __builtin_nontemporal_load(&a[i]);
+  else v = a[i];
+  v = v * 1.0000001f;
+  if constexpr (NT) __builtin_nontemporal_store(v, &a[i]);
+  else a[i] = v;
+}
 }  // namespace
+
+extern "C" int m3lb_stream_probe(uint64_t bytes, int32_t nontemporal, int32_t reps,
+                                 double* best_ms) {
+  if (!best_ms || bytes < 16 || reps < 1) return fail(M3LB_EINVAL, "bad argument");
+  const uint64_t n4 = bytes / 16;
+  f4v* a = nullptr;
+  if (hipMalloc(&a, n4 * 16) != hipSuccess) return fail(M3LB_ENOMEM, "stream probe: hipMalloc failed");
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = M3LB_OK;
+  if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 0, n4 * 16, st) != hipSuccess)
+    rc = fail(M3LB_EDEVICE, "stream probe: setup failed");
+  const unsigned grid = unsigned((n4 + m3lb::kBlock - 1) / m3lb::kBlock);
+  double best = 1e30;
+  for (int r = 0; r < reps && !rc; ++r) {
+    hipEventRecord(e0, st);
+    if (nontemporal) k_stream<true><<<grid, m3lb::kBlock, 0, st>>>(a, n4);
+    else k_stream<false><<<grid, m3lb::kBlock, 0, st>>>(a, n4);
+    hipEventRecord(e1, st);
+    rc = launch_ok();
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(M3LB_EDEVICE, "stream probe: sync failed");
+    float ms = 0;
+    if (!rc) hipEventElapsedTime(&ms, e0, e1);
+    if (!rc && double(ms) < best) best = double(ms);
+  }
+  if (st) hipStreamSynchronize(st);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  hipFree(a);
+  if (rc) return rc;
+  *best_ms = best;
+  return M3LB_OK;
+}
 
 extern "C" int m3lb_energy_enstrophy(m3lb_solver* s, double* energy, double* enstrophy,
                                      int32_t* finite) {
@@ -650,6 +707,18 @@ int m3lb_set_scales(m3lb_solver* s, const double* sc) {
     if (sc[q] != sc[o]) return fail(M3LB_EINVAL, "opposite directions must share a scale");
   }
   return apply_scales(s, sc);
+}
+
+int m3lb_set_tile_order(m3lb_solver* s, int32_t run) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  s->remap = run;
+  return M3LB_OK;
+}
+
+int m3lb_tile_order_map(uint32_t grid, int32_t run, uint32_t* out) {
+  if (!out) return fail(M3LB_EINVAL, "null argument");
+  for (uint32_t b = 0; b < grid; ++b) out[b] = m3lb::tile_of(b, grid, run);
+  return M3LB_OK;
 }
 
 int m3lb_device_bytes(const m3lb_solver* s, uint64_t* bytes) {

@@ -48,7 +48,28 @@ SIGNATURES = {
     "m3lb_calibrate_scales": (ctypes.c_int, [_vp]),
     "m3lb_time_steps": (ctypes.c_int, [_vp, ctypes.c_int32, _dp]),
     "m3lb_device_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "m3lb_set_tile_order": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "m3lb_tile_order_map": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int32,
+                                           ctypes.POINTER(ctypes.c_uint32)]),
+    "m3lb_stream_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _dp]),
 }
+
+
+def stream_probe(nbytes, nontemporal=True, reps=5):
+    """m3lb_stream_probe: best milliseconds for an in-place stream of nbytes."""
+    L = load()
+    ms = ctypes.c_double()
+    _check(L.m3lb_stream_probe(int(nbytes), 1 if nontemporal else 0, int(reps), ctypes.byref(ms)))
+    return ms.value
+
+
+def tile_order_map(grid, run):
+    """m3lb_tile_order_map: the step kernel's block -> tile map."""
+    L = load()
+    out = np.zeros(int(grid), dtype=np.uint32)
+    _check(L.m3lb_tile_order_map(int(grid), int(run),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+    return out
 
 
 def load():
@@ -211,6 +232,9 @@ class FluidSolver:
 
     def calibrate_scales(self):
         _check(self.L.m3lb_calibrate_scales(self.h))
+
+    def set_tile_order(self, run):
+        _check(self.L.m3lb_set_tile_order(self.h, int(run)))
 
     def device_bytes(self):
         b = ctypes.c_uint64()

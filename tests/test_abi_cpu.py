@@ -92,3 +92,13 @@ def test_null_arguments_are_rejected(lib):
     assert lib.m3lb_create(None, None) == abi.EINVAL
     assert lib.m3lb_step(None, 1, 0) == abi.EINVAL
     assert lib.m3lb_destroy(None) == abi.OK
+
+
+@pytest.mark.parametrize("run", [0, -1, 1, 2, 3, 7, 16, 64, 512])
+def test_tile_order_is_a_permutation(lib, run):
+    """Every step launch must visit each tile exactly once, for every grid
+    size (including grids not divisible by 8 or by 8 * run)."""
+    import numpy as np
+    for grid in list(range(1, 70)) + [80, 127, 520, 1000, 4096 + 8 * 17 + 3, 524288]:
+        m = solver.tile_order_map(grid, run)
+        assert np.array_equal(np.sort(m), np.arange(grid, dtype=np.uint32)), (grid, run)

@@ -1,0 +1,41 @@
+"""The C++ facade (include/m3lb/lbm.hpp) on the device: the Poiseuille program
+of tests/cpp/poiseuille_facade.cpp, written as the reference's
+validation/poiseuille.cpp drives FluidSolver, reproduces the reference's
+README slip table (validation/poiseuille.cpp:203-218)."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from golden_cases import GOLD
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "poiseuille_facade")
+
+
+def test_facade_poiseuille_matches_reference_table():
+    print("\n[gpu-case] facade poiseuille", flush=True)
+    sys.stdout.flush()
+    assert os.path.exists(EXE), "run __graft_entry__.build() first"
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for line in r.stdout.split("\n"):
+        if line.strip():
+            op, tau, l2, slip, steps = line.split()
+            got[(op, round(float(tau), 6))] = (float(l2), float(slip))
+    g = GOLD["poiseuille_slip"]
+    for row in g["rows"]:
+        tau = 0.5 + 3 ** 0.5 / 4 if abs(row[0] - 0.9330127) < 1e-7 else row[0]
+        for col in ("BGK", "TRT", "MRT", "CM"):
+            key = (col, round(tau, 6))
+            if key not in got:
+                continue
+            want = row[g["columns"].index(col)]
+            _, slip = got[key]
+            if abs(want) < 1e-11:
+                assert abs(slip) < 1e-11, (key, slip)
+            else:
+                assert abs(slip - want) <= 0.0051 * abs(want), (key, slip, want)
+    assert len(got) == 12

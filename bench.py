@@ -74,6 +74,101 @@ def cpu_leg(D, steps, threads):
     return D ** 3 * steps / sec / 1e6, sec
 
 
+def _pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), q)) if v else None
+
+
+def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk):
+    """The requests through one batched server: arrivals replayed on the
+    server clock, continuous admission, completion times from the server."""
+    from m3lb_amd import abi
+    from m3lb_amd.serve import Server
+    scfg = abi.make_server_config(budget_bytes=budget_bytes, max_boxes=4096,
+                                  steps_per_chunk=steps_per_chunk)
+    n = len(work)
+    with Server(scfg) as srv:
+        tune = srv.autotune(steps=tune_steps) if tune_steps > 0 else None
+        log(f"[bench] serving autotune: {tune}")
+        t0 = srv.clock() + 0.01
+        nxt, ttc, last, peak_res = 0, [], t0, 0
+        while len(ttc) < n:
+            now = srv.clock()
+            while nxt < n and t0 + work[nxt][0] <= now:
+                req = work[nxt][1]
+                req.arrival = t0 + work[nxt][0]
+                req.keep = 0
+                srv.submit(req)
+                nxt += 1
+            if nxt < n:
+                wait = int(min(0.005, max(0.0, t0 + work[nxt][0] - srv.clock())) * 1e6)
+                srv.progress(wait_us=max(wait, 1))
+            else:
+                srv.progress(wait_us=-1)
+            for r in srv.poll():
+                ttc.append(r["completed"] - r["arrival"])
+                last = max(last, r["completed"])
+            peak_res = max(peak_res, srv.stats()["resident"])
+        st = srv.stats()
+    lu = sum(r.nx * r.ny * r.nz * r.steps for _, r in work)
+    span = last - t0
+    return {"mlups": lu / span / 1e6, "makespan_s": span, "p50_s": _pct(ttc, 50),
+            "p99_s": _pct(ttc, 99), "device_mlups": st["lattice_updates"] / st["device_seconds"] / 1e6,
+            "peak_resident": peak_res, "peak_bytes": st["bytes_peak"], "chunks": st["chunks"],
+            "steps_per_chunk": steps_per_chunk, "tune": tune, "lattice_updates": lu}
+
+
+def serving_per_case(work):
+    """The same requests one case at a time, in arrival order, each on its own
+    single-box solver: create, geometry, initialise, one launch per step,
+    read the mass (synchronises), destroy."""
+    from m3lb_amd import abi, cases
+    from m3lb_amd.solver import FluidSolver
+    scfg = abi.make_server_config()
+    t0 = time.perf_counter() + 0.01
+    free, ttc, step_s = t0, [], 0.0
+    for off, r in work:
+        arr = t0 + off
+        now = time.perf_counter()
+        if now < arr:
+            time.sleep(arr - now)
+        with FluidSolver(abi.request_config(scfg, r)) as s:
+            if r.walls == abi.WALLS_Y:
+                s.set_geometry(cases.channel_flags((r.nx, r.ny, r.nz)))
+            if r.init == abi.INIT_TAYLOR_GREEN:
+                s.initialize_taylor_green(r.u[0])
+            else:
+                s.initialize(r.rho0, *r.u)
+            step_s += s.time_steps(r.steps) / 1e3
+            s.total_mass()
+        free = time.perf_counter()
+        ttc.append(free - arr)
+    lu = sum(r.nx * r.ny * r.nz * r.steps for _, r in work)
+    span = free - t0
+    return {"mlups": lu / span / 1e6, "makespan_s": span, "p50_s": _pct(ttc, 50),
+            "p99_s": _pct(ttc, 99), "step_only_mlups": lu / step_s / 1e6}
+
+
+def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk):
+    from m3lb_amd import cases
+    work = cases.serving_workload(n, seed=seed, rate=rate)
+    b = serving_batched(work, int(budget_gb * 1e9), tune_steps, steps_per_chunk)
+    log(f"[bench] serving batched: {b['mlups']:.0f} MLUPS, p50 {b['p50_s']:.2f} s, "
+        f"p99 {b['p99_s']:.2f} s, makespan {b['makespan_s']:.1f} s, peak resident "
+        f"{b['peak_resident']}")
+    work = cases.serving_workload(n, seed=seed, rate=rate)
+    c = serving_per_case(work)
+    log(f"[bench] serving one launch per case: {c['mlups']:.0f} MLUPS, p50 {c['p50_s']:.2f} s, "
+        f"p99 {c['p99_s']:.2f} s, makespan {c['makespan_s']:.1f} s")
+    rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+    return {"requests": n, "arrival_rate_per_s": rate, "seed": seed,
+            "workload": "D3Q27 CM FP32 shifted; 50% periodic Taylor-Green u0=0.04, 50% "
+                        "bounce-back channels g_x~U[1e-7,1e-5]; edges log-uniform [32,192) "
+                        "per axis; tau~U[0.51,0.8]; steps~U{200..3000}; Poisson arrivals",
+            "budget_bytes": int(budget_gb * 1e9), "batched": rnd(b), "per_case": rnd(c),
+            "speedup_mlups": round(b["mlups"] / c["mlups"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +182,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true",
                     help="also time the 16-bit and FP64 storage forms and BGK (stderr)")
+    ap.add_argument("--serving-requests", type=int, default=200,
+                    help="requests of the batched-serving leg (0 skips it)")
+    ap.add_argument("--serving-rate", type=float, default=40.0, help="arrivals per second")
+    ap.add_argument("--serving-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serving-chunk", type=int, default=8, help="steps per chunk")
+    ap.add_argument("--serving-tune-steps", type=int, default=3)
+    ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--stream-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain in-place stream of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -105,6 +207,11 @@ def main():
         sys.exit(2)
 
     from m3lb_amd import abi
+    if args.serving_only:
+        sv = serving_leg(args.serving_requests, args.serving_rate, 4, args.serving_budget_gb,
+                         args.serving_tune_steps, args.serving_chunk)
+        print(json.dumps({"serving": sv}), flush=True)
+        return
     dt = {"f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "f64": abi.F64}[args.dtype]
     r = gpu_leg(args.size, args.steps, args.warmup, dt, abi.OP_CM)
     log(f"[bench] gpu {args.size}^3 CM {args.dtype}: {r['mlups']:.0f} MLUPS, "
@@ -151,8 +258,13 @@ def main():
             traffic = json.load(fh)["traffic_bytes"]
         traffic_src = "profiles/k_step_cm_f32_512cubed.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
+    serving = None
+    if args.serving_requests > 0:
+        serving = serving_leg(args.serving_requests, args.serving_rate, 4,
+                              args.serving_budget_gb, args.serving_tune_steps, args.serving_chunk)
+
     out = {
-        "metric": "D3Q27 CM MLUPS + %HBM roofline at 512^3",
+        "metric": "D3Q27 CM MLUPS + %HBM roofline at 512^3; batched-solve p50/p99 latency",
         "value": round(r["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -177,6 +289,7 @@ def main():
         "stream_ceiling_gbs": ceil,
         "frac_of_stream_ceiling": round(r["achieved_gbs"] / best_ceiling, 4),
         "mass_drift": r["mass_drift"],
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -157,6 +157,14 @@ int m3lb_storage_sum_exact(m3lb_solver* s, int64_t out[2]);
 int m3lb_energy_enstrophy(m3lb_solver* s, double* energy, double* enstrophy,
                           int32_t* finite);
 
+/* Momentum exchange with a body (validation/square_cylinder.cpp:27-50,
+ * 340-350): over every link from a Fluid node n to a neighbour n + c_i in the
+ * body, F += 2 c_i f(n, opp(i)), f(n, opp(i)) being the population arriving
+ * at n along opp(i) (gather_populations, unshifted).  body: one byte per
+ * INTERIOR node, x fastest, nonzero = in the body; NULL = every Solid node.
+ * FP64 accumulation; force = {Fx, Fy, Fz}.  The reference evaluates the same
+ * sum on the host from gather_populations. */
+int m3lb_momentum_exchange(m3lb_solver* s, const uint8_t* body, double force[3]);
 /* gather_populations() / seed_populations(fn) (FluidSolver.hpp:625-637,
  * 1231-1253): the logical population travelling in direction q that arrives
  * at node n, (n, q) ordered, n_padded * Q doubles.  Seeding writes each slot
@@ -215,6 +223,120 @@ int m3lb_stream_probe(uint64_t bytes, int32_t nontemporal, int32_t reps, double*
 
 /* Device bytes held by the handle (populations + flags + macroscopic). */
 int m3lb_device_bytes(const m3lb_solver* s, uint64_t* bytes);
+/* initialize_field with the Taylor-Green vortex of
+ * validation/tgv3d_bench.cpp:79-90 evaluated on the device: w = 0,
+ *   u = u0 sin X cos Y cos Z,  v = -u0 cos X sin Y cos Z,
+ *   rho = 1 + 3 u0^2/16 (cos 2X + cos 2Y)(cos 2Z + 2),
+ * X = 2 pi (x - hx) / nx and likewise Y, Z with their own extents (for a cube,
+ * the reference's field).  Non-fluid nodes at rest with rho = 1. */
+int m3lb_initialize_taylor_green(m3lb_solver* s, double u0);
+
+/* ------------------------------------------------------------------------
+ * Batched serving.  No counterpart in the reference.  A server keeps many
+ * independent boxes (requests) resident in one device arena of a fixed byte
+ * budget and advances all of them together: each step is one persistent
+ * kernel over the packed tile list of every resident box; a chunk of
+ * steps_per_chunk steps is one hipGraph launch.  Requests are admitted first
+ * come, first served as slots and memory free up (a bounded number of smaller
+ * requests may pass a queue head that does not fit yet), and complete
+ * independently.  A box computes bit-identical results alone, in any mix, and
+ * in the single-box solver with the same configuration and forcing policy.
+ * Every box of a server shares one kernel configuration (lattice, operator,
+ * storage, dtype, forcing policy); F32 and F64 storage only.
+ * One host thread per server; calls are ordered on the server's own stream.
+ * ------------------------------------------------------------------------ */
+enum { M3LB_SHAPE_ROWS = 0, M3LB_SHAPE_LINEAR = 1, M3LB_SHAPE_AUTO = 2 };
+typedef struct m3lb_server_config {
+  int32_t lattice, op, storage, dtype; /* as in m3lb_config */
+  int32_t forcing;         /* M3LB_FORCE_NONE | M3LB_FORCE_GUO (force per request) */
+  int32_t max_boxes;       /* resident slots, 1 .. 65536 */
+  int32_t steps_per_chunk; /* steps per launch sequence, 1 .. 1024 */
+  int32_t use_graph;       /* 1: a full chunk is one hipGraph launch */
+  int32_t blocks_per_cu;   /* persistent grid = CUs * blocks_per_cu, 1 .. 64 */
+  int32_t tile_run;        /* XCD order of the packed tiles (m3lb_set_tile_order) */
+  int32_t tile_shape;      /* M3LB_SHAPE_*: ROWS = whole x-rows per tile (as the
+                              single-box step), LINEAR = 256 consecutive
+                              interior nodes, AUTO = ROWS when nx % 64 == 0 */
+  int32_t max_bypass;      /* admissions allowed past a waiting queue head */
+  uint64_t budget_bytes;   /* device arena for resident boxes */
+} m3lb_server_config;
+enum { M3LB_INIT_UNIFORM = 0, M3LB_INIT_TAYLOR_GREEN = 1 };
+enum { M3LB_WALLS_NONE = 0, M3LB_WALLS_Y = 1 };
+typedef struct m3lb_request {
+  int32_t nx, ny, nz;
+  int32_t periodic[3];
+  int32_t init;            /* M3LB_INIT_UNIFORM (rho0, u) or M3LB_INIT_TAYLOR_GREEN
+                              (amplitude u[0], as m3lb_initialize_taylor_green) */
+  int32_t walls;           /* M3LB_WALLS_Y: interior rows y = 0, ny-1 Solid;
+                              ignored when flags are passed to submit */
+  int32_t steps;           /* >= 1 */
+  int32_t keep;            /* 1: the box stays resident after completion for
+                              the fetch calls, until m3lb_srv_release */
+  double omega, omega_minus, omega_bulk;
+  double force[3];         /* Guo force (servers with M3LB_FORCE_GUO) */
+  double rho0, u[3];
+  double arrival;          /* server-clock seconds; < 0: the time of submit */
+} m3lb_request;
+enum { M3LB_REQ_QUEUED = 0, M3LB_REQ_RESIDENT = 1, M3LB_REQ_DONE = 2, M3LB_REQ_CANCELLED = 3 };
+typedef struct m3lb_result {
+  int64_t id;
+  int32_t state;           /* M3LB_REQ_* */
+  int32_t steps;           /* steps completed */
+  double arrival, admitted, completed; /* server-clock seconds */
+  double mass, momentum[3]; /* at completion: sums over fluid nodes, FP64 */
+} m3lb_result;
+typedef struct m3lb_server_stats {
+  int64_t submitted, rejected, queued, resident, done, cancelled;
+  int64_t chunks, steps;   /* chunks and batched steps issued */
+  int64_t lattice_updates; /* interior nodes x steps of retired chunks */
+  uint64_t bytes_used, bytes_peak, bytes_budget;
+  double device_seconds;   /* retired chunks, event-timed on the stream */
+  int32_t grid, tile_run, tile_shape, steps_per_chunk;
+} m3lb_server_stats;
+typedef struct m3lb_tune_result {
+  int32_t tile_shape, tile_run, blocks_per_cu, candidates;
+  double best_glups;       /* chosen setting on the tuning mix */
+  double initial_glups;    /* the server's setting before tuning */
+} m3lb_tune_result;
+typedef struct m3lb_server m3lb_server;
+/* Allocates the arena (budget_bytes) on the current device. */
+int m3lb_srv_create(const m3lb_server_config* cfg, m3lb_server** out);
+int m3lb_srv_destroy(m3lb_server* srv);
+/* Arena bytes one request needs (populations + flags); no device call. */
+int m3lb_srv_footprint(const m3lb_server_config* cfg, const m3lb_request* req,
+                       uint64_t* bytes);
+/* Queues a request.  interior_flags (nx*ny*nz bytes, x fastest, values as
+ * m3lb_set_geometry) may be NULL.  A request larger than the whole budget is
+ * refused with M3LB_EINVAL; one that does not fit now waits in the queue. */
+int m3lb_srv_submit(m3lb_server* srv, const m3lb_request* req,
+                    const uint8_t* interior_flags, int64_t* id);
+/* Retires finished chunks, admits queued requests, issues the next chunks
+ * (at most two in flight).  wait_us > 0 waits up to that long for a chunk to
+ * retire when none has, < 0 waits for one, 0 returns at once.
+ * *n_completed (may be NULL) = requests completed by this call. */
+int m3lb_srv_progress(m3lb_server* srv, int32_t wait_us, int32_t* n_completed);
+/* Runs progress until no request is queued or resident-and-unfinished. */
+int m3lb_srv_drain(m3lb_server* srv);
+/* Pops up to max records of requests completed or cancelled since the last
+ * poll.  A popped record of a request without keep is forgotten. */
+int m3lb_srv_poll(m3lb_server* srv, m3lb_result* out, int32_t max, int32_t* n);
+int m3lb_srv_result(m3lb_server* srv, int64_t id, m3lb_result* out);
+/* Fields of a completed request submitted with keep = 1, as
+ * m3lb_compute_macroscopic / m3lb_gather_populations of its box. */
+int m3lb_srv_fetch_macroscopic(m3lb_server* srv, int64_t id, double* rho, double* ux,
+                               double* uy, double* uz);
+int m3lb_srv_fetch_populations(m3lb_server* srv, int64_t id, double* out);
+/* Frees a kept box and forgets its request. */
+int m3lb_srv_release(m3lb_server* srv, int64_t id);
+/* Removes a queued or resident request; its memory returns to the arena. */
+int m3lb_srv_cancel(m3lb_server* srv, int64_t id);
+int m3lb_srv_stats(m3lb_server* srv, m3lb_server_stats* out);
+int m3lb_srv_clock(m3lb_server* srv, double* seconds);
+/* Times every tile shape x tile run x blocks-per-CU candidate on a fixed mix
+ * of Taylor-Green boxes (edges log-uniform on [32, 192]) for `steps` steps
+ * each and keeps the fastest.  Only on an idle server (no request queued or
+ * resident).  Results of later requests do not depend on the choice. */
+int m3lb_srv_autotune(m3lb_server* srv, int32_t steps, m3lb_tune_result* out);
 
 #ifdef __cplusplus
 }

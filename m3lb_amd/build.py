@@ -1,7 +1,8 @@
 """Builds libm3lb_hip.so for gfx950 with hipcc.
 
 Twelve instantiation units (one per lattice x collision operator, from
-csrc/inst.hip) and the C-ABI unit compile in parallel into build objects under
+csrc/inst.hip), the C-ABI unit of the single-box solver (csrc/solver.hip) and
+the batched server (csrc/serve.hip) compile in parallel into build objects under
 m3lb_amd/_build/, then link into m3lb_amd/libm3lb_hip.so.  Every HIP line
 carries -Xclang -target-feature -Xclang -packed-fp32-ops: hipcc then emits no
 packed FP32 instructions.
@@ -58,10 +59,11 @@ def build(force=True, jobs=None, verbose=False):
         if force or _stale(o, deps):
             cmds.append([HIPCC] + FLAGS + [f"-DM3LB_Q={q}", f"-DM3LB_OP={op}", "-c",
                                            os.path.join(CSRC, "inst.hip"), "-o", o])
-    o = os.path.join(OBJ, "solver.o")
-    objs.append(o)
-    if force or _stale(o, deps):
-        cmds.append([HIPCC] + FLAGS + ["-c", os.path.join(CSRC, "solver.hip"), "-o", o])
+    for unit in ("solver", "serve"):
+        o = os.path.join(OBJ, f"{unit}.o")
+        objs.append(o)
+        if force or _stale(o, deps):
+            cmds.append([HIPCC] + FLAGS + ["-c", os.path.join(CSRC, f"{unit}.hip"), "-o", o])
     # largest units first
     cmds.sort(key=lambda c: ("q27" not in c[-1], c[-1]))
     with ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -231,3 +231,103 @@ def poiseuille_run(solver, cfg, H, nu, G, max_steps=600000, probe_every=200):
     rho, ux, _, _ = solver.compute_macroscopic()
     l2, rm, slip = poiseuille_measure(cfg, H, nu, G, rho, ux)
     return l2, rm, slip, taken
+
+
+def taylor_green_box(cfg, u0):
+    """Host copy of m3lb_initialize_taylor_green: the tgv3d_bench field
+    (validation/tgv3d_bench.cpp:79-90) with one wavenumber per axis,
+    k_a = 2 pi / n_a (for a cube, tgv3d_benchmark(cfg, u0))."""
+    x, y, z, d = padded_coords(cfg)
+    X = (2.0 * math.pi / cfg.nx) * (x - d[3])
+    Y = (2.0 * math.pi / cfg.ny) * (y - d[4])
+    Z = (2.0 * math.pi / cfg.nz) * (z - d[5])
+    dr = 3.0 * u0 * u0 / 16.0
+    rho = 1.0 + dr * (np.cos(2.0 * X) + np.cos(2.0 * Y)) * (np.cos(2.0 * Z) + 2.0)
+    ux = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    uy = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    return rho, ux, uy, np.zeros_like(ux)
+
+
+def channel_flags(shape):
+    """Interior flags of the bounce-back channel: rows y = 0 and y = ny - 1
+    Solid (validation/poiseuille.cpp:86-90), x fastest."""
+    nx, ny, nz = shape
+    fl = np.zeros((nz, ny, nx), dtype=np.uint8)
+    fl[:, 0, :] = abi.SOLID
+    fl[:, ny - 1, :] = abi.SOLID
+    return fl.reshape(-1)
+
+
+def serving_workload(n, seed=4, rate=40.0, lattice=abi.D3Q27, u0=0.04):
+    """The batched-serving workload: n independent requests arriving as a
+    Poisson stream of `rate` per second (seeded).  Half are periodic
+    Taylor-Green boxes (amplitude u0, m3lb_initialize_taylor_green), half
+    bounce-back channels (walls at y = 0 and ny - 1, periodic x and z, Guo
+    force g_x ~ U[1e-7, 1e-5], starting at rest).  Each edge is drawn
+    separately, log-uniform on [32, 192); tau ~ U[0.51, 0.8]; steps ~ U{200..3000}.
+    Returns a list of (arrival offset in seconds, abi.Request)."""
+    rng = np.random.Generator(np.random.MT19937(seed))
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate)
+        kind = rng.random() < 0.5
+        e = [int(32.0 * 6.0 ** rng.random()) for _ in range(3)]
+        if lattice == abi.D2Q9:
+            e[2] = 1
+        tau = rng.uniform(0.51, 0.8)
+        steps = int(rng.integers(200, 3001))
+        g = rng.uniform(1e-7, 1e-5)
+        if kind:
+            r = abi.make_request(e, steps, omega=1.0 / tau, init=abi.INIT_TAYLOR_GREEN,
+                                 u=(u0, 0.0, 0.0))
+        else:
+            r = abi.make_request(e, steps, omega=1.0 / tau, periodic=(True, False, True),
+                                 walls=abi.WALLS_Y, force=(g, 0.0, 0.0))
+        out.append((t, r))
+    return out
+
+
+def sphere_channel(shape, n_spheres=200, rmin=6.0, rmax=14.0, seed=2, max_tries=200000):
+    """Channel with spheres: Solid layers y in {0, ny-1} and z in {0, nz-1},
+    plus n_spheres spheres with radii r ~ U[rmin, rmax] whose centres are
+    drawn by seeded rejection sampling, accepted when the sphere keeps one
+    free layer to every wall and to every accepted sphere.  A node is in a
+    sphere when |x - c| <= r.  Returns (interior flags x fastest, sphere mask
+    x fastest, list of (cx, cy, cz, r))."""
+    nx, ny, nz = shape
+    rng = np.random.Generator(np.random.MT19937(seed))
+    spheres = []
+    tries = 0
+    while len(spheres) < n_spheres and tries < max_tries:
+        tries += 1
+        r = rng.uniform(rmin, rmax)
+        c = (rng.uniform(0, nx), rng.uniform(r + 2.0, ny - 3.0 - r), rng.uniform(r + 2.0, nz - 3.0 - r))
+        if c[1] > ny - 3.0 - r or c[2] > nz - 3.0 - r:
+            continue
+        ok = True
+        for (a, b, e, q) in spheres:
+            dx = abs(c[0] - a)
+            dx = min(dx, nx - dx)  # periodic in x
+            if dx * dx + (c[1] - b) ** 2 + (c[2] - e) ** 2 < (r + q + 1.0) ** 2:
+                ok = False
+                break
+        if ok:
+            spheres.append((c[0], c[1], c[2], r))
+    body = np.zeros((nz, ny, nx), dtype=bool)
+    for (a, b, e, q) in spheres:
+        z0, z1 = int(math.floor(e - q)), int(math.ceil(e + q))
+        y0, y1 = int(math.floor(b - q)), int(math.ceil(b + q))
+        zz, yy = np.meshgrid(np.arange(z0, z1 + 1), np.arange(y0, y1 + 1), indexing="ij")
+        xs = np.arange(int(math.floor(a - q)), int(math.ceil(a + q)) + 1)
+        d2 = ((xs[None, None, :] - a) ** 2 + (yy[:, :, None] - b) ** 2 + (zz[:, :, None] - e) ** 2)
+        inside = d2 <= q * q
+        kz, ky, kx = np.nonzero(inside)
+        body[zz[kz, ky], yy[kz, ky], xs[kx] % nx] = True
+    flags = np.zeros((nz, ny, nx), dtype=np.uint8)
+    flags[:, 0, :] = abi.SOLID
+    flags[:, ny - 1, :] = abi.SOLID
+    flags[0, :, :] = abi.SOLID
+    flags[nz - 1, :, :] = abi.SOLID
+    flags[body] = abi.SOLID
+    return flags.reshape(-1), body.reshape(-1).astype(np.uint8), spheres

@@ -8,6 +8,7 @@
 // the same block.
 #pragma once
 
+#include "fastdiv.hpp"
 #include "lbm_device.hpp"
 
 namespace m3lb {
@@ -194,45 +195,144 @@ __host__ __device__ __forceinline__ unsigned tile_of(unsigned b, unsigned G, int
   return whole * 8u + xcd * (m - whole) + (k - whole);
 }
 
-// run_step<P, StoreMacro> (FluidSolver.hpp:712-1038), Fluid and Solid paths:
-// solid and excluded nodes do nothing (implicit bounce-back, :759-761).  One
-// thread per node; a tile is 256 threads covering either one 256-wide x-chunk
-// of a row (nx > 256) or 256/bx whole rows with bx >= nx.
-template <class K, int P, bool GEOM>
-__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
-  using R = typename K::R;
-  using T = typename K::T;
-  const unsigned tile = tile_of(blockIdx.x, gridDim.x, a.remap);
-  int row, x;
-  if (a.chunks > 1) {
-    row = int(tile / unsigned(a.chunks));
-    x = int(tile - unsigned(row) * unsigned(a.chunks)) * kBlock + int(threadIdx.x);
+// Row and x of the node a thread updates in tile `tile` of a box: one
+// 256-wide x-chunk of a row when chunks > 1, else 256/bx whole rows (bx >= nx).
+__device__ __forceinline__ void tile_node(unsigned tile, int chunks, int bx_log2, int& row,
+                                          int& x) {
+  if (chunks > 1) {
+    row = int(tile / unsigned(chunks));
+    x = int(tile - unsigned(row) * unsigned(chunks)) * kBlock + int(threadIdx.x);
   } else {
-    row = int(tile) * (kBlock >> a.bx_log2) + int(threadIdx.x >> a.bx_log2);
-    x = int(threadIdx.x & ((1u << a.bx_log2) - 1u));
-  }
-  const Dom& d = a.d;
-  if (row >= a.rows || x >= d.nx) return;
-  const int y = row % d.ny;
-  const int z = row / d.ny;
-  const unsigned n = unsigned(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + d.hx + x);
-  if constexpr (GEOM) {
-    if (a.flags[n] != M3LB_FLUID) return;
-  }
-  const Params<R> p = params_of<K>(a);
-  T* __restrict__ A = static_cast<T*>(a.f);
-  const Offs o = offsets(d, x, y, z);
-  R f[K::Q];
-  load_node<K, P>(A, a.stride, n, o, a.inv_scale, f);
-  const Macro<R> m = collide_fused<K>(f, p);
-  store_node<K, P>(A, a.stride, n, o, a.scale, f);
-  if (a.macro[0]) {
-    static_cast<R*>(a.macro[0])[n] = density<K>(m);
-    static_cast<R*>(a.macro[1])[n] = m.ux;
-    static_cast<R*>(a.macro[2])[n] = m.uy;
-    static_cast<R*>(a.macro[3])[n] = m.uz;
+    row = int(tile) * (kBlock >> bx_log2) + int(threadIdx.x >> bx_log2);
+    x = int(threadIdx.x & ((1u << bx_log2) - 1u));
   }
 }
+
+// The per-node body of run_step<P, StoreMacro> (FluidSolver.hpp:712-1038),
+// Fluid and Solid paths: solid and excluded nodes do nothing (implicit
+// bounce-back, :759-761); a fluid node loads its incoming set, collides and
+// stores it back crossed.  Shared by the single-box and the batched kernels,
+// so a box computes bit-identical results in either.
+template <class K, int P, bool GEOM>
+__device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint64_t stride,
+                                            const uint8_t* flags, const Dom& d, int x, int y,
+                                            int z, const Params<typename K::R>& p,
+                                            const float* scale, const float* inv_scale,
+                                            void* const* macro) {
+  using R = typename K::R;
+  const unsigned n = unsigned(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + d.hx + x);
+  if constexpr (GEOM) {
+    if (flags[n] != M3LB_FLUID) return;
+  }
+  const Offs o = offsets(d, x, y, z);
+  R f[K::Q];
+  load_node<K, P>(A, stride, n, o, inv_scale, f);
+  const Macro<R> m = collide_fused<K>(f, p);
+  store_node<K, P>(A, stride, n, o, scale, f);
+  if (macro && macro[0]) {
+    static_cast<R*>(macro[0])[n] = density<K>(m);
+    static_cast<R*>(macro[1])[n] = m.ux;
+    static_cast<R*>(macro[2])[n] = m.uy;
+    static_cast<R*>(macro[3])[n] = m.uz;
+  }
+}
+
+// run_step<P, StoreMacro> over one box: one thread per node, tiles of 256.
+template <class K, int P, bool GEOM>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+  int row, x;
+  tile_node(tile_of(blockIdx.x, gridDim.x, a.remap), a.chunks, a.bx_log2, row, x);
+  const Dom& d = a.d;
+  if (row >= a.rows || x >= d.nx) return;
+  const Params<typename K::R> p = params_of<K>(a);
+  node_update<K, P, GEOM>(static_cast<typename K::T*>(a.f), a.stride, a.flags, d, x,
+                          row % d.ny, row / d.ny, p, a.scale, a.inv_scale, a.macro);
+}
+
+// ---------------------------------------------------------------------------
+// Batched step: every resident box of a server advances in one launch.  The
+// tiles of all boxes form one packed list (tile_start = prefix sum of the
+// boxes' tile counts); a persistent grid of G blocks (G a multiple of 8) walks
+// it in virtual-tile order v = block + k*G, mapped through tile_of so that
+// runs of consecutive tiles stay on one XCD.  Step j of a chunk of steps is
+// launch j; box b takes part while j < steps, at parity (parity0 + j) & 1.  A
+// box's step barrier is the kernel boundary.  Box count and tile total are
+// read from device memory, so one captured sequence of launches serves every
+// chunk.  The per-node work is node_update, as in k_step, so a box gets
+// bit-identical results alone, in any mix of boxes, and in the single-box
+// solver.
+// ---------------------------------------------------------------------------
+enum TileShape { kShapeRows = 0, kShapeLinear = 1 };
+
+struct BoxDesc {
+  void* f;
+  uint64_t stride;
+  const uint8_t* flags;
+  Dom d;
+  int shape;        // kShapeRows: tile_node's rows / x-chunks; kShapeLinear:
+                    // 256 consecutive interior nodes in x-fastest order
+  int chunks, bx_log2, rows;
+  unsigned nodes;   // nx * ny * nz
+  FastDiv div_x, div_y, div_xy;
+  int parity0, steps;
+  double omega, omega_m, omega_b, F[3];
+  float scale[27], inv_scale[27];
+};
+
+struct BatchArgs {
+  const BoxDesc* desc;
+  const unsigned* hdr;  // {boxes, total tiles, tile_start[0..boxes]}
+  int j;                // step within the chunk
+  int run;              // tile order, see tile_of()
+};
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  const unsigned nbox = a.hdr[0], total = a.hdr[1];
+  const unsigned* start = a.hdr + 2;
+  for (unsigned v = blockIdx.x; v < total; v += gridDim.x) {
+    const unsigned t = tile_of(v, total, a.run);
+    unsigned lo = 0, hi = nbox;  // largest lo with start[lo] <= t
+    while (hi - lo > 1) {
+      const unsigned mid = (lo + hi) >> 1;
+      if (start[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const BoxDesc& b = a.desc[lo];
+    if (a.j >= b.steps) continue;
+    const unsigned local = t - start[lo];
+    int x, y, z;
+    if (b.shape == kShapeLinear) {
+      const unsigned i = local * unsigned(kBlock) + threadIdx.x;
+      if (i >= b.nodes) continue;
+      const unsigned zz = fd_div(b.div_xy, i);
+      const unsigned r = i - zz * (unsigned(b.d.nx) * unsigned(b.d.ny));
+      const unsigned yy = fd_div(b.div_x, r);
+      x = int(r - yy * unsigned(b.d.nx));
+      y = int(yy);
+      z = int(zz);
+    } else {
+      int row;
+      tile_node(local, b.chunks, b.bx_log2, row, x);
+      if (row >= b.rows || x >= b.d.nx) continue;
+      z = int(fd_div(b.div_y, unsigned(row)));
+      y = row - z * b.d.ny;
+    }
+    Params<R> p;
+    p.omega = R(b.omega); p.omega_m = R(b.omega_m); p.omega_b = R(b.omega_b);
+    p.F[0] = R(b.F[0]); p.F[1] = R(b.F[1]); p.F[2] = R(b.F[2]);
+    T* A = static_cast<T*>(b.f);
+    // Flags are always checked: a second (flag-free) variant of the node body
+    // next to the two parities would take the kernel from 67 to 113 VGPRs.
+    if (((b.parity0 + a.j) & 1) == 0)
+      node_update<K, 0, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
+    else
+      node_update<K, 1, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // Seeding.  The reference's scatter<P> (FluidSolver.hpp:1231-1241) walks the
@@ -242,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 // what remains.  Here every slot is visited once and takes the value of its
 // highest-indexed writer, which is that same result with no write conflict.
 // ---------------------------------------------------------------------------
-enum SeedMode { kSeedUniform = 0, kSeedField = 1, kSeedArray = 2 };
+enum SeedMode { kSeedUniform = 0, kSeedField = 1, kSeedArray = 2, kSeedTaylorGreen = 3 };
 
 struct SeedArgs {
   void* f;
@@ -273,6 +373,23 @@ __device__ __forceinline__ int writer_axis(int m, int e, int n, int s, int per) 
   return best;
 }
 
+// Taylor-Green vortex of validation/tgv3d_bench.cpp:79-90 (w = 0, the
+// pressure-carrying density) at padded node n, with one wavenumber per axis,
+// k_a = 2 pi / n_a, so that boxes with unequal edges stay periodic; for a cube
+// it is the reference's field.
+__device__ __forceinline__ void taylor_green_at(const Dom& d, int n, double u0, double& rho,
+                                                double& ux, double& uy) {
+  const int x = n % d.sx, y = (n / d.sx) % d.sy, z = n / (d.sx * d.sy);
+  const double pi2 = 6.283185307179586;
+  const double X = (pi2 / d.nx) * double(x - d.hx);
+  const double Y = (pi2 / d.ny) * double(y - d.hy);
+  const double Z = (pi2 / d.nz) * double(z - d.hz);
+  const double dr = 3.0 * u0 * u0 / 16.0;
+  rho = 1.0 + dr * (cos(2.0 * X) + cos(2.0 * Y)) * (cos(2.0 * Z) + 2.0);
+  ux = u0 * sin(X) * cos(Y) * cos(Z);
+  uy = -u0 * cos(X) * sin(Y) * cos(Z);
+}
+
 template <class K>
 __device__ __forceinline__ void seed_value_of(const SeedArgs& a, int n, typename K::R* v) {
   using R = typename K::R;
@@ -280,6 +397,14 @@ __device__ __forceinline__ void seed_value_of(const SeedArgs& a, int n, typename
   if (a.mode == kSeedUniform) {
     if (fluid) seed<K>(v, R(a.rho0), R(a.u0[0]), R(a.u0[1]), R(a.u0[2]));
     else seed<K>(v, R(a.rho0), R(0), R(0), R(0));
+  } else if (a.mode == kSeedTaylorGreen) {
+    if (fluid) {
+      double rho, ux, uy;
+      taylor_green_at(a.d, n, a.u0[0], rho, ux, uy);
+      seed<K>(v, R(rho), R(ux), R(uy), R(0));
+    } else {
+      seed<K>(v, R(1), R(0), R(0), R(0));
+    }
   } else {
     if (fluid)
       seed<K>(v, R(a.field[0][n]), R(a.field[1][n]), R(a.field[2][n]), R(a.field[3][n]));
@@ -374,34 +499,41 @@ __device__ __forceinline__ int shift_axis(int v, int h, int n, int s, int per) {
   return v;
 }
 
-template <class K>
-__global__ void __launch_bounds__(kBlock) k_gather(ReadArgs a) {
-  using R = typename K::R;
+// Stored value of the population travelling in direction i that arrives at
+// padded node n = (x, y, z): Access<P>::load's slot for (n, i).
+template <class K, int i>
+__device__ __forceinline__ typename K::R arriving(const ReadArgs& a, int n, int x, int y, int z) {
   using T = typename K::T;
   using C = typename K::C;
+  constexpr int Q = K::Q;
+  const Dom& d = a.d;
+  const T* A = static_cast<const T*>(a.f);
+  int node = n, slot = i;
+  if constexpr (i != 0) {
+    if constexpr ((i & 1) == 0) {  // read from the odd partner's neighbour
+      constexpr int j = i - 1;
+      const int nx_ = shift_axis(x + Lat<Q>::cx[j], d.hx, d.nx, d.sx, d.px);
+      const int ny_ = shift_axis(y + Lat<Q>::cy[j], d.hy, d.ny, d.sy, d.py);
+      const int nz_ = shift_axis(z + Lat<Q>::cz[j], d.hz, d.nz, d.sz, d.pz);
+      node = (nz_ * d.sy + ny_) * d.sx + nx_;
+      slot = a.parity == 0 ? i : i - 1;
+    } else {
+      slot = a.parity == 0 ? i : i + 1;
+    }
+  }
+  return C::dec(A[uint64_t(slot) * a.stride + unsigned(node)], a.inv_scale[slot]);
+}
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_gather(ReadArgs a) {
   constexpr int Q = K::Q;
   const int n = blockIdx.x * kBlock + threadIdx.x;
   const Dom& d = a.d;
   if (n >= d.np) return;
   const int x = n % d.sx, y = (n / d.sx) % d.sy, z = n / (d.sx * d.sy);
-  const T* A = static_cast<const T*>(a.f);
   sfor<Q>([&](auto I) {
     constexpr int i = I;
-    int node = n, slot = i;
-    if constexpr (i != 0) {
-      if constexpr ((i & 1) == 0) {  // read from the odd partner's neighbour
-        constexpr int j = i - 1;
-        const int nx_ = shift_axis(x + Lat<Q>::cx[j], d.hx, d.nx, d.sx, d.px);
-        const int ny_ = shift_axis(y + Lat<Q>::cy[j], d.hy, d.ny, d.sy, d.py);
-        const int nz_ = shift_axis(z + Lat<Q>::cz[j], d.hz, d.nz, d.sz, d.pz);
-        node = (nz_ * d.sy + ny_) * d.sx + nx_;
-        slot = a.parity == 0 ? i : i - 1;
-      } else {
-        slot = a.parity == 0 ? i : i + 1;
-      }
-    }
-    const R v = C::dec(A[uint64_t(slot) * a.stride + unsigned(node)], a.inv_scale[slot]);
-    a.out[uint64_t(n) * Q + i] = double(v);
+    a.out[uint64_t(n) * Q + i] = double(arriving<K, i>(a, n, x, y, z));
   });
 }
 
@@ -516,6 +648,41 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mass(ReadArgs a) {
     acc[3] += double(r * m.uz);
   }
   block_sum<4>(acc, a.out);
+}
+
+// Momentum exchange with a body (validation/square_cylinder.cpp:27-50,
+// 340-350): over every link from a Fluid node n to a neighbour n + c_i (the
+// reference's neighbour rule) that belongs to the body,
+//     F += 2 c_i f(n, opp(i)),
+// f(n, opp(i)) being the population arriving at n along opp(i) -- the
+// post-collision f_i that halfway bounce-back returned -- unshifted in the
+// compute type as unshift<C> does.  body: padded bytes, nonzero = in the body.
+// Three FP64 partials per block.
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_momentum_exchange(ReadArgs a, const uint8_t* body) {
+  using R = typename K::R;
+  constexpr int Q = K::Q;
+  const Dom& d = a.d;
+  double acc[3] = {0, 0, 0};
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < d.np; n += gridDim.x * kBlock) {
+    if (a.flags[n] != M3LB_FLUID) continue;
+    const int x = n % d.sx, y = (n / d.sx) % d.sy, z = n / (d.sx * d.sy);
+    sfor<Q - 1>([&](auto I) {
+      constexpr int i = int(I) + 1;
+      constexpr int io = opp<Q>(i);
+      const int mx = shift_axis(x + Lat<Q>::cx[i], d.hx, d.nx, d.sx, d.px);
+      const int my = shift_axis(y + Lat<Q>::cy[i], d.hy, d.ny, d.sy, d.py);
+      const int mz = shift_axis(z + Lat<Q>::cz[i], d.hz, d.nz, d.sz, d.pz);
+      if (!body[(mz * d.sy + my) * d.sx + mx]) return;
+      R f = arriving<K, io>(a, n, x, y, z);
+      if constexpr (K::SHIFTED) f = f + weight<K>(io);
+      const double fd = double(f);
+      acc[0] += 2.0 * Lat<Q>::cx[i] * fd;
+      acc[1] += 2.0 * Lat<Q>::cy[i] * fd;
+      acc[2] += 2.0 * Lat<Q>::cz[i] * fd;
+    });
+  }
+  block_sum<3>(acc, a.out);
 }
 
 // Exact sum of every slot: value * 2^100 truncated toward zero, summed as a

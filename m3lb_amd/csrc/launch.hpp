@@ -20,6 +20,9 @@ struct KernelSet {
   void (*sum_exact)(const ReadArgs&, unsigned grid, hipStream_t);
   void (*absmax)(const ReadArgs&, unsigned grid, hipStream_t);
   void (*rescale)(void* f, uint64_t stride, int np, const Ratio&, unsigned grid, hipStream_t);
+  void (*exchange)(const ReadArgs&, const uint8_t* body, unsigned grid, hipStream_t);
+  // batched server step; null for the identity op and 16-bit storage
+  void (*batched)(const BatchArgs&, unsigned grid, hipStream_t);
   int elem_bytes;    // bytes per stored population
   int real_bytes;    // bytes of the compute type
 };
@@ -28,6 +31,7 @@ inline unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBl
 
 template <class K>
 struct Launch {
+  static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::C::scaled;
   static void step(const StepArgs& a, int parity, bool geom, unsigned grid, hipStream_t s) {
     if (parity == 0) {
       if (geom) k_step<K, 0, true><<<grid, kBlock, 0, s>>>(a);
@@ -62,13 +66,27 @@ struct Launch {
                       hipStream_t s) {
     if constexpr (K::C::scaled) k_rescale<K><<<grid, kBlock, 0, s>>>(f, stride, np, r);
   }
+  static void exchange(const ReadArgs& a, const uint8_t* body, unsigned grid, hipStream_t s) {
+    k_momentum_exchange<K><<<grid, kBlock, 0, s>>>(a, body);
+  }
+  static void batched(const BatchArgs& a, unsigned grid, hipStream_t s) {
+    if constexpr (kBatched) k_step_batched<K><<<grid, kBlock, 0, s>>>(a);
+  }
   static const KernelSet* table() {
     static const KernelSet t = {step,     seed,      gather, storage,
                                 macro,    reduce_mass, sum_exact, absmax,
-                                rescale,  int(sizeof(typename K::T)), int(sizeof(typename K::R))};
+                                rescale,  exchange,
+                                kBatched ? batched : nullptr,
+                                int(sizeof(typename K::T)), int(sizeof(typename K::R))};
     return &t;
   }
 };
+
+// Kernel table of a runtime configuration, or null (defined with the C-ABI of
+// the single-box solver).
+const KernelSet* kernel_set(int q, int op, int storage, int dtype, int force);
+// Records msg as the calling thread's m3lb_last_error() and returns code.
+int set_error(int code, const char* msg);
 
 }  // namespace m3lb
 

@@ -200,6 +200,14 @@ int calibrate(m3lb_solver* s);
 
 }  // namespace
 
+namespace m3lb {
+const KernelSet* kernel_set(int q, int op, int storage, int dtype, int force) {
+  if (q != M3LB_D2Q9 && q != M3LB_D3Q27) return nullptr;
+  return lookup(q, op, storage, dtype, force);
+}
+int set_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace m3lb
+
 extern "C" {
 
 const char* m3lb_last_error(void) { return g_err.c_str(); }
@@ -315,6 +323,23 @@ int m3lb_initialize(m3lb_solver* s, double rho0, double ux, double uy, double uz
   a.mode = m3lb::kSeedUniform;
   a.rho0 = rho0;
   a.u0[0] = ux; a.u0[1] = uy; a.u0[2] = uz;
+  s->ks->seed(a, s->stream);
+  int rc = launch_ok();
+  if (rc) return rc;
+  rc = sync(s);
+  if (rc) return rc;
+  return is16(s) ? calibrate(s) : M3LB_OK;
+}
+
+int m3lb_initialize_taylor_green(m3lb_solver* s, double u0) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  if (!std::isfinite(u0)) return fail(M3LB_EINVAL, "amplitude must be finite");
+  s->t = 0;
+  s->macro_valid = false;
+  provisional_scales(s);
+  m3lb::SeedArgs a = seed_args(s);
+  a.mode = m3lb::kSeedTaylorGreen;
+  a.u0[0] = u0;
   s->ks->seed(a, s->stream);
   int rc = launch_ok();
   if (rc) return rc;
@@ -477,6 +502,46 @@ int m3lb_total_momentum(m3lb_solver* s, int32_t axis, double* out) {
   int rc = reduce4(s, r);
   if (rc) return rc;
   *out = r[1 + axis];
+  return M3LB_OK;
+}
+
+int m3lb_momentum_exchange(m3lb_solver* s, const uint8_t* body, double force[3]) {
+  if (!s || !force) return fail(M3LB_EINVAL, "null argument");
+  const m3lb::Dom& d = s->d;
+  std::vector<uint8_t> pad(size_t(d.np), 0);
+  if (body) {
+    size_t k = 0;
+    for (int z = 0; z < d.nz; ++z)
+      for (int y = 0; y < d.ny; ++y)
+        for (int x = 0; x < d.nx; ++x, ++k)
+          pad[size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx)] = body[k] ? 1 : 0;
+  } else {
+    for (size_t n = 0; n < pad.size(); ++n) pad[n] = s->flags_h[n] == M3LB_SOLID ? 1 : 0;
+  }
+  const unsigned grid = reduce_grid(s);
+  int rc = ensure_partials(s, size_t(grid) * 3);
+  if (rc) return rc;
+  uint8_t* body_d = nullptr;
+  HIP_OK(hipMalloc(&body_d, pad.size()));
+  rc = M3LB_OK;
+  if (hipMemcpyAsync(body_d, pad.data(), pad.size(), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    rc = fail(M3LB_EDEVICE, "body upload failed");
+  std::vector<double> h(size_t(grid) * 3);
+  if (!rc) {
+    m3lb::ReadArgs a = read_args(s);
+    a.out = s->partial_d;
+    s->ks->exchange(a, body_d, grid, s->stream);
+    rc = launch_ok();
+  }
+  if (!rc && hipMemcpyAsync(h.data(), s->partial_d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+    rc = fail(M3LB_EDEVICE, "partials download failed");
+  const hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(body_d);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(M3LB_EDEVICE, hipGetErrorString(e));
+  for (int k = 0; k < 3; ++k) force[k] = 0;
+  for (unsigned b = 0; b < grid; ++b)
+    for (int k = 0; k < 3; ++k) force[k] += h[size_t(b) * 3 + k];
   return M3LB_OK;
 }
 

@@ -52,6 +52,28 @@ SIGNATURES = {
     "m3lb_tile_order_map": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int32,
                                            ctypes.POINTER(ctypes.c_uint32)]),
     "m3lb_stream_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _dp]),
+    "m3lb_initialize_taylor_green": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m3lb_momentum_exchange": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8), _dp]),
+    # batched server
+    "m3lb_srv_create": (ctypes.c_int, [ctypes.POINTER(abi.ServerConfig), ctypes.POINTER(_vp)]),
+    "m3lb_srv_destroy": (ctypes.c_int, [_vp]),
+    "m3lb_srv_footprint": (ctypes.c_int, [ctypes.POINTER(abi.ServerConfig),
+                                          ctypes.POINTER(abi.Request),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "m3lb_srv_submit": (ctypes.c_int, [_vp, ctypes.POINTER(abi.Request),
+                                       ctypes.POINTER(ctypes.c_uint8), _i64p]),
+    "m3lb_srv_progress": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "m3lb_srv_drain": (ctypes.c_int, [_vp]),
+    "m3lb_srv_poll": (ctypes.c_int, [_vp, ctypes.POINTER(abi.Result), ctypes.c_int32,
+                                     ctypes.POINTER(ctypes.c_int32)]),
+    "m3lb_srv_result": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.POINTER(abi.Result)]),
+    "m3lb_srv_fetch_macroscopic": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp, _dp, _dp]),
+    "m3lb_srv_fetch_populations": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
+    "m3lb_srv_release": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "m3lb_srv_cancel": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "m3lb_srv_stats": (ctypes.c_int, [_vp, ctypes.POINTER(abi.ServerStats)]),
+    "m3lb_srv_clock": (ctypes.c_int, [_vp, _dp]),
+    "m3lb_srv_autotune": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(abi.TuneResult)]),
 }
 
 
@@ -146,6 +168,11 @@ class FluidSolver:
     def initialize(self, rho0, ux=0.0, uy=0.0, uz=0.0):
         _check(self.L.m3lb_initialize(self.h, rho0, ux, uy, uz))
 
+    def initialize_taylor_green(self, u0):
+        """m3lb_initialize_taylor_green: the tgv3d_bench vortex evaluated on the
+        device (cases.taylor_green_box gives the same field on the host)."""
+        _check(self.L.m3lb_initialize_taylor_green(self.h, float(u0)))
+
     def initialize_field(self, rho, ux, uy, uz):
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (rho, ux, uy, uz)]
         for a in arrs:
@@ -197,6 +224,18 @@ class FluidSolver:
         _check(self.L.m3lb_energy_enstrophy(self.h, ctypes.byref(e), ctypes.byref(w),
                                             ctypes.byref(ok)))
         return e.value, w.value, bool(ok.value)
+
+    def momentum_exchange(self, body=None):
+        """m3lb_momentum_exchange: (Fx, Fy, Fz) on the body (interior mask,
+        x fastest; None = every Solid node)."""
+        f = (ctypes.c_double * 3)()
+        bp = None
+        if body is not None:
+            b = np.ascontiguousarray(body, dtype=np.uint8).reshape(-1)
+            assert b.size == self.cfg.nx * self.cfg.ny * self.cfg.nz
+            bp = b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        _check(self.L.m3lb_momentum_exchange(self.h, bp, f))
+        return tuple(f)
 
     def gather_populations(self):
         out = np.zeros(self.np * self.Q)

@@ -114,3 +114,48 @@ def tgv3d_run(solver_cls, probes, dtype=abi.F64, **kw):
         assert ok
         out.append((tmax * k * probe / T, e / e0, w / w0))
     return out
+
+
+def momentum_exchange_np(cfg, flags, body, gathered):
+    """validation/square_cylinder.cpp:340-350 over every Fluid node: for each
+    link (n, i) whose neighbour n + c_i (periodic wrap, else clamped into the
+    padded box) is in the body, F += 2 c_i f(n, opp(i)), f = the gathered
+    (stored) value unshifted in the compute type (unshift<C>).  flags and body
+    are interior arrays, x fastest; gathered is (n_padded, Q)."""
+    from m3lb_amd import lattice
+    d = abi.domain_of(cfg)
+    nx, ny, nz, hx, hy, hz, sx, sy, sz, npad = d
+    Q = cfg.lattice
+    c = np.asarray(lattice.velocities(Q), dtype=np.int64)
+    w = np.asarray(lattice.weights(Q))
+    ct = np.float64 if cfg.dtype == abi.F64 else np.float32
+    shifted = cfg.storage == abi.SHIFTED and cfg.op != abi.OP_NULL
+    pf = np.full((sz, sy, sx), abi.EXCLUDED, dtype=np.uint8)
+    pb = np.zeros((sz, sy, sx), dtype=bool)
+    pf[hz:hz + nz, hy:hy + ny, hx:hx + nx] = np.asarray(flags).reshape(nz, ny, nx)
+    pb[hz:hz + nz, hy:hy + ny, hx:hx + nx] = np.asarray(body).reshape(nz, ny, nx) != 0
+    zz, yy, xx = np.nonzero(pf == abi.FLUID)
+    n = (zz * sy + yy) * sx + xx
+
+    def shift(v, h, m, s, per):
+        if per:
+            return np.where(v < h, v + m, np.where(v >= h + m, v - m, v))
+        return np.clip(v, 0, s - 1)
+
+    F = np.zeros(3)
+    per = cfg.periodic
+    for i in range(1, Q):
+        mx = shift(xx + c[i, 0], hx, nx, sx, per[0])
+        my = shift(yy + c[i, 1], hy, ny, sy, per[1])
+        mz = shift(zz + c[i, 2] if c.shape[1] > 2 else zz, hz, nz, sz, per[2])
+        hit = pb[mz, my, mx]
+        if not hit.any():
+            continue
+        io = lattice.opp(i)
+        f = gathered[n[hit], io].astype(ct)
+        if shifted:
+            f = (f + ct(w[io])).astype(ct)
+        f = f.astype(np.float64)
+        for a in range(min(3, c.shape[1])):
+            F[a] += np.sum(2.0 * c[i, a] * f)
+    return F

@@ -36,7 +36,7 @@ def taylor_green_config(abi, D, dtype, op, storage=None):
 
 
 def bytes_per_node(abi, dtype, Q=27):
-    return 2 * Q * {abi.F64: 8, abi.F32: 4, abi.F16: 2, abi.BF16: 2}[dtype]
+    return 2 * Q * {abi.F64: 8, abi.F32: 4, abi.F16: 2, abi.BF16: 2, abi.I16: 2}[dtype]
 
 
 def gpu_leg(D, steps, warmup, dtype, op, seed=3):
@@ -79,13 +79,14 @@ def _pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else None
 
 
-def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk):
+def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk, max_resident_nodes=0):
     """The requests through one batched server: arrivals replayed on the
     server clock, continuous admission, completion times from the server."""
     from m3lb_amd import abi
     from m3lb_amd.serve import Server
     scfg = abi.make_server_config(budget_bytes=budget_bytes, max_boxes=4096,
-                                  steps_per_chunk=steps_per_chunk)
+                                  steps_per_chunk=steps_per_chunk,
+                                  max_resident_nodes=max_resident_nodes)
     n = len(work)
     with Server(scfg) as srv:
         tune = srv.autotune(steps=tune_steps) if tune_steps > 0 else None
@@ -115,7 +116,8 @@ def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk):
     return {"mlups": lu / span / 1e6, "makespan_s": span, "p50_s": _pct(ttc, 50),
             "p99_s": _pct(ttc, 99), "device_mlups": st["lattice_updates"] / st["device_seconds"] / 1e6,
             "peak_resident": peak_res, "peak_bytes": st["bytes_peak"], "chunks": st["chunks"],
-            "steps_per_chunk": steps_per_chunk, "tune": tune, "lattice_updates": lu}
+            "steps_per_chunk": steps_per_chunk, "max_resident_nodes": max_resident_nodes,
+            "tune": tune, "lattice_updates": lu}
 
 
 def serving_per_case(work):
@@ -149,10 +151,47 @@ def serving_per_case(work):
             "p99_s": _pct(ttc, 99), "step_only_mlups": lu / step_s / 1e6}
 
 
-def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk):
+def precision_study(D, steps, seed=3):
+    """The Taylor-Green workload stored in FP64, FP32, FP16 and BF16 from the
+    same initial field: velocity relative L2 and kinetic-energy difference of
+    each against the FP64-storage run after `steps` steps, and MLUPS."""
+    import numpy as np
+    from m3lb_amd import abi, cases
+    from m3lb_amd.solver import FluidSolver
+    cfg0, u0 = taylor_green_config(abi, D, abi.F64, abi.OP_CM)
+    field = cases.tgv3d_benchmark(cfg0, u0, perturbation=1e-4, seed=seed)
+    ref, out = None, {}
+    for name, dt in (("f64", abi.F64), ("f32", abi.F32), ("f16", abi.F16), ("bf16", abi.BF16),
+                     ("i16", abi.I16)):
+        cfg, _ = taylor_green_config(abi, D, dt, abi.OP_CM)
+        with FluidSolver(cfg) as s:
+            s.initialize_field(*field)
+            e0, _, _ = s.energy_enstrophy()
+            ms = s.time_steps(steps)
+            _, ux, uy, uz = s.compute_macroscopic()
+            e1, w1, fin = s.energy_enstrophy()
+            scales = s.scales() if dt in (abi.F16, abi.BF16, abi.I16) else None
+        row = {"mlups": round(D ** 3 * steps / (ms / 1e3) / 1e6, 1), "energy": e1,
+               "energy_ratio_to_initial": e1 / e0, "enstrophy": w1, "finite": bool(fin)}
+        if ref is None:
+            ref = (ux, uy, uz, e1)
+        else:
+            num = float(np.sum((ux - ref[0]) ** 2 + (uy - ref[1]) ** 2 + (uz - ref[2]) ** 2))
+            den = float(np.sum(ref[0] ** 2 + ref[1] ** 2 + ref[2] ** 2))
+            row["vel_rel_l2_vs_f64"] = (num / den) ** 0.5
+            row["energy_rel_diff_vs_f64"] = abs(e1 - ref[3]) / ref[3]
+        if scales is not None:
+            row["scale_log2"] = [int(round(math.log2(v))) for v in scales]
+        out[name] = row
+        log(f"[bench] precision {name}: {row}")
+        del ux, uy, uz
+    return {"size": D, "steps": steps, "seed": seed, "rows": out}
+
+
+def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0):
     from m3lb_amd import cases
     work = cases.serving_workload(n, seed=seed, rate=rate)
-    b = serving_batched(work, int(budget_gb * 1e9), tune_steps, steps_per_chunk)
+    b = serving_batched(work, int(budget_gb * 1e9), tune_steps, steps_per_chunk, cap)
     log(f"[bench] serving batched: {b['mlups']:.0f} MLUPS, p50 {b['p50_s']:.2f} s, "
         f"p99 {b['p99_s']:.2f} s, makespan {b['makespan_s']:.1f} s, peak resident "
         f"{b['peak_resident']}")
@@ -175,7 +214,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "i16", "f64"])
     ap.add_argument("--cpu-size", type=int, default=192)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -188,7 +227,11 @@ def main():
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     ap.add_argument("--serving-chunk", type=int, default=8, help="steps per chunk")
     ap.add_argument("--serving-tune-steps", type=int, default=3)
+    ap.add_argument("--serving-cap", type=int, default=0,
+                    help="max resident interior nodes of the server (0: memory only)")
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
+                    help="only run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
     ap.add_argument("--stream-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain in-place stream of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -207,12 +250,17 @@ def main():
         sys.exit(2)
 
     from m3lb_amd import abi
+    if args.precision_study:
+        print(json.dumps({"precision": precision_study(args.size, args.precision_study)}),
+              flush=True)
+        return
     if args.serving_only:
         sv = serving_leg(args.serving_requests, args.serving_rate, 4, args.serving_budget_gb,
-                         args.serving_tune_steps, args.serving_chunk)
+                         args.serving_tune_steps, args.serving_chunk, args.serving_cap)
         print(json.dumps({"serving": sv}), flush=True)
         return
-    dt = {"f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "f64": abi.F64}[args.dtype]
+    dt = {"f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "i16": abi.I16,
+          "f64": abi.F64}[args.dtype]
     r = gpu_leg(args.size, args.steps, args.warmup, dt, abi.OP_CM)
     log(f"[bench] gpu {args.size}^3 CM {args.dtype}: {r['mlups']:.0f} MLUPS, "
         f"{r['ms_per_step']:.3f} ms/step, {r['achieved_gbs']:.0f} GB/s, "
@@ -220,6 +268,7 @@ def main():
 
     if args.extra:
         for name, d2, op in (("f16", abi.F16, abi.OP_CM), ("bf16", abi.BF16, abi.OP_CM),
+                             ("i16", abi.I16, abi.OP_CM),
                              ("f64", abi.F64, abi.OP_CM), ("f32-bgk", abi.F32, abi.OP_BGK)):
             x = gpu_leg(args.size, max(10, args.steps // 2), 5, d2, op)
             log(f"[bench] extra {name}: {x['mlups']:.0f} MLUPS {x['achieved_gbs']:.0f} GB/s "
@@ -261,7 +310,8 @@ def main():
     serving = None
     if args.serving_requests > 0:
         serving = serving_leg(args.serving_requests, args.serving_rate, 4,
-                              args.serving_budget_gb, args.serving_tune_steps, args.serving_chunk)
+                              args.serving_budget_gb, args.serving_tune_steps, args.serving_chunk,
+                              args.serving_cap)
 
     out = {
         "metric": "D3Q27 CM MLUPS + %HBM roofline at 512^3; batched-solve p50/p99 latency",
@@ -275,7 +325,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"f32": "f32", "f16": "f16-storage/f32", "bf16": "bf16-storage/f32",
-                  "f64": "f64"}[args.dtype],
+                  "i16": "i16-storage/f32", "f64": "f64"}[args.dtype],
         "data": "synthetic",
         "config": {"workload": f"D3Q27 central-moment esoteric-pull step, {args.dtype} shifted "
                                f"populations, {args.size}^3 periodic Taylor-Green u0=0.04 "

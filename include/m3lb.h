@@ -58,11 +58,13 @@ enum {
  * src/memory/Storage.hpp:27-35. */
 enum { M3LB_RAW = 0, M3LB_SHIFTED = 1 };
 
-/* Storage dtype of the population array.  F64 computes in FP64; F32, F16 and
- * BF16 compute in FP32.  F16/BF16 hold s_q * g_q with one scale per direction
- * and require SHIFTED storage.  The reference selects FP32/FP64 with a
- * configure-time typedef (src/core/Types.hpp:16-20) and has no 16-bit form. */
-enum { M3LB_F64 = 0, M3LB_F32 = 1, M3LB_F16 = 2, M3LB_BF16 = 3 };
+/* Storage dtype of the population array.  F64 computes in FP64; F32 and the
+ * 16-bit forms compute in FP32.  The 16-bit forms hold s_q * g_q with one
+ * power-of-two scale per opposite pair and require SHIFTED storage: F16 and
+ * BF16 as floating point, I16 as fixed point (round to nearest, saturating at
+ * +-32767).  The reference selects FP32/FP64 with a configure-time typedef
+ * (src/core/Types.hpp:16-20) and has no 16-bit form. */
+enum { M3LB_F64 = 0, M3LB_F32 = 1, M3LB_F16 = 2, M3LB_BF16 = 3, M3LB_I16 = 4 };
 
 /* Forcing policy: NoForcing or uniform Guo (src/forcing/Forcing.hpp:41-82). */
 enum { M3LB_FORCE_NONE = 0, M3LB_FORCE_GUO = 1 };
@@ -190,8 +192,10 @@ int m3lb_set_omega(m3lb_solver* s, double omega, double omega_minus, double omeg
 int m3lb_get_scales(m3lb_solver* s, double* scales_q);
 int m3lb_set_scales(m3lb_solver* s, const double* scales_q);
 /* Re-chooses the 16-bit scales from the current field: per opposite pair the
- * largest power of two keeping max |g_q| * s_q below 2^14.  Runs after every
- * initialize; callers may repeat it on a schedule.  No-op for F32/F64. */
+ * power of two that maps max |g_q| into [2^7, 2^8) (256x headroom under
+ * FP16's largest finite value; a pair below 1e-3 w_q counts as 1e-3 w_q).
+ * Runs after every initialize; callers may repeat it on a schedule.  No-op
+ * for F32/F64. */
 int m3lb_calibrate_scales(m3lb_solver* s);
 
 /* Runs nsteps steps bracketed by HIP events recorded on the handle's stream
@@ -246,6 +250,7 @@ int m3lb_initialize_taylor_green(m3lb_solver* s, double u0);
  * One host thread per server; calls are ordered on the server's own stream.
  * ------------------------------------------------------------------------ */
 enum { M3LB_SHAPE_ROWS = 0, M3LB_SHAPE_LINEAR = 1, M3LB_SHAPE_AUTO = 2 };
+enum { M3LB_TILE_BLOCK_RANGES = -2 };
 typedef struct m3lb_server_config {
   int32_t lattice, op, storage, dtype; /* as in m3lb_config */
   int32_t forcing;         /* M3LB_FORCE_NONE | M3LB_FORCE_GUO (force per request) */
@@ -253,12 +258,21 @@ typedef struct m3lb_server_config {
   int32_t steps_per_chunk; /* steps per launch sequence, 1 .. 1024 */
   int32_t use_graph;       /* 1: a full chunk is one hipGraph launch */
   int32_t blocks_per_cu;   /* persistent grid = CUs * blocks_per_cu, 1 .. 64 */
-  int32_t tile_run;        /* XCD order of the packed tiles (m3lb_set_tile_order) */
+  int32_t tile_run;        /* order of the packed tiles: >= -1 as in
+                              m3lb_set_tile_order (one box lookup per tile), or
+                              M3LB_TILE_BLOCK_RANGES (one contiguous range of
+                              the packed list per block) */
   int32_t tile_shape;      /* M3LB_SHAPE_*: ROWS = whole x-rows per tile (as the
                               single-box step), LINEAR = 256 consecutive
                               interior nodes, AUTO = ROWS when nx % 64 == 0 */
   int32_t max_bypass;      /* admissions allowed past a waiting queue head */
   uint64_t budget_bytes;   /* device arena for resident boxes */
+  uint64_t max_resident_nodes; /* admission stops while the resident boxes hold
+                              this many interior nodes (one box is always
+                              admitted into an empty server); 0 = no limit.
+                              Bounds the work shared per step, so a request's
+                              time to completion stays near its own run time
+                              once the device is saturated. */
 } m3lb_server_config;
 enum { M3LB_INIT_UNIFORM = 0, M3LB_INIT_TAYLOR_GREEN = 1 };
 enum { M3LB_WALLS_NONE = 0, M3LB_WALLS_Y = 1 };
@@ -292,6 +306,7 @@ typedef struct m3lb_server_stats {
   uint64_t bytes_used, bytes_peak, bytes_budget;
   double device_seconds;   /* retired chunks, event-timed on the stream */
   int32_t grid, tile_run, tile_shape, steps_per_chunk;
+  uint64_t resident_nodes; /* interior nodes of the resident boxes */
 } m3lb_server_stats;
 typedef struct m3lb_tune_result {
   int32_t tile_shape, tile_run, blocks_per_cu, candidates;
@@ -332,7 +347,7 @@ int m3lb_srv_release(m3lb_server* srv, int64_t id);
 int m3lb_srv_cancel(m3lb_server* srv, int64_t id);
 int m3lb_srv_stats(m3lb_server* srv, m3lb_server_stats* out);
 int m3lb_srv_clock(m3lb_server* srv, double* seconds);
-/* Times every tile shape x tile run x blocks-per-CU candidate on a fixed mix
+/* Times every tile shape x tile order x blocks-per-CU candidate on a fixed mix
  * of Taylor-Green boxes (edges log-uniform on [32, 192]) for `steps` steps
  * each and keeps the fastest.  Only on an idle server (no request queued or
  * resident).  Results of later requests do not depend on the choice. */

@@ -189,7 +189,13 @@ inline void check(int rc, const char* what) {
     throw std::runtime_error(std::string(what) + ": " + m3lb_last_error());
 }
 
-enum class Precision : int { FP64 = M3LB_F64, FP32 = M3LB_F32, FP16 = M3LB_F16, BF16 = M3LB_BF16 };
+enum class Precision : int {
+  FP64 = M3LB_F64,
+  FP32 = M3LB_F32,
+  FP16 = M3LB_F16,
+  BF16 = M3LB_BF16,
+  I16 = M3LB_I16
+};
 
 #if defined(LBM_SINGLE_PRECISION)
 constexpr Precision kDefaultPrecision = Precision::FP32;

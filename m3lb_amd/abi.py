@@ -23,6 +23,7 @@ F64 = 0
 F32 = 1
 F16 = 2
 BF16 = 3
+I16 = 4
 
 FORCE_NONE = 0
 FORCE_GUO = 1
@@ -39,7 +40,7 @@ ESTATE = -4
 
 OP_NAMES = {OP_NULL: "null", OP_BGK: "bgk", OP_BGK_PRODUCT: "bgk_product",
             OP_TRT: "trt", OP_MRT: "mrt", OP_CM: "cm"}
-DTYPE_NAMES = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16"}
+DTYPE_NAMES = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16", I16: "i16"}
 
 
 class Config(ctypes.Structure):
@@ -113,6 +114,8 @@ SHAPE_ROWS = 0
 SHAPE_LINEAR = 1
 SHAPE_AUTO = 2
 
+TILE_BLOCK_RANGES = -2
+
 INIT_UNIFORM = 0
 INIT_TAYLOR_GREEN = 1
 
@@ -142,6 +145,7 @@ class ServerConfig(ctypes.Structure):
         ("tile_shape", ctypes.c_int32),
         ("max_bypass", ctypes.c_int32),
         ("budget_bytes", ctypes.c_uint64),
+        ("max_resident_nodes", ctypes.c_uint64),
     ]
 
 
@@ -203,6 +207,7 @@ class ServerStats(ctypes.Structure):
         ("tile_run", ctypes.c_int32),
         ("tile_shape", ctypes.c_int32),
         ("steps_per_chunk", ctypes.c_int32),
+        ("resident_nodes", ctypes.c_uint64),
     ]
 
 
@@ -221,7 +226,8 @@ class TuneResult(ctypes.Structure):
 
 def make_server_config(lattice=D3Q27, op=OP_CM, storage=SHIFTED, dtype=F32, forcing=FORCE_GUO,
                        budget_bytes=1 << 30, max_boxes=4096, steps_per_chunk=8, use_graph=True,
-                       blocks_per_cu=8, tile_run=64, tile_shape=SHAPE_AUTO, max_bypass=64):
+                       blocks_per_cu=8, tile_run=64, tile_shape=SHAPE_AUTO, max_bypass=64,
+                       max_resident_nodes=0):
     c = ServerConfig()
     c.lattice, c.op, c.storage, c.dtype, c.forcing = lattice, op, storage, dtype, forcing
     c.max_boxes = int(max_boxes)
@@ -232,6 +238,7 @@ def make_server_config(lattice=D3Q27, op=OP_CM, storage=SHIFTED, dtype=F32, forc
     c.tile_shape = int(tile_shape)
     c.max_bypass = int(max_bypass)
     c.budget_bytes = int(budget_bytes)
+    c.max_resident_nodes = int(max_resident_nodes)
     return c
 
 

@@ -40,6 +40,7 @@ M3LB_EXPAND_LOOKUP(M3LB_Q, M3LB_OP) {
   if (dtype == M3LB_F32) return force ? pick<M3LB_SHIFTED, M3LB_F32, true>() : pick<M3LB_SHIFTED, M3LB_F32, false>();
   if (dtype == M3LB_F16) return force ? pick<M3LB_SHIFTED, M3LB_F16, true>() : pick<M3LB_SHIFTED, M3LB_F16, false>();
   if (dtype == M3LB_BF16) return force ? pick<M3LB_SHIFTED, M3LB_BF16, true>() : pick<M3LB_SHIFTED, M3LB_BF16, false>();
+  if (dtype == M3LB_I16) return force ? pick<M3LB_SHIFTED, M3LB_I16, true>() : pick<M3LB_SHIFTED, M3LB_I16, false>();
   return nullptr;
 #endif
 }

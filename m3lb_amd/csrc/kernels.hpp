@@ -286,12 +286,72 @@ struct BatchArgs {
   int run;              // tile order, see tile_of()
 };
 
+// One tile of box b at step j of the chunk.
 template <class K>
-__global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
+__device__ __forceinline__ void batched_tile(const BoxDesc& b, unsigned local, int j) {
   using R = typename K::R;
   using T = typename K::T;
+  int x, y, z;
+  if (b.shape == kShapeLinear) {
+    const unsigned i = local * unsigned(kBlock) + threadIdx.x;
+    if (i >= b.nodes) return;
+    const unsigned zz = fd_div(b.div_xy, i);
+    const unsigned r = i - zz * (unsigned(b.d.nx) * unsigned(b.d.ny));
+    const unsigned yy = fd_div(b.div_x, r);
+    x = int(r - yy * unsigned(b.d.nx));
+    y = int(yy);
+    z = int(zz);
+  } else {
+    int row;
+    tile_node(local, b.chunks, b.bx_log2, row, x);
+    if (row >= b.rows || x >= b.d.nx) return;
+    z = int(fd_div(b.div_y, unsigned(row)));
+    y = row - z * b.d.ny;
+  }
+  Params<R> p;
+  p.omega = R(b.omega); p.omega_m = R(b.omega_m); p.omega_b = R(b.omega_b);
+  p.F[0] = R(b.F[0]); p.F[1] = R(b.F[1]); p.F[2] = R(b.F[2]);
+  T* A = static_cast<T*>(b.f);
+  // Flags are always checked: a second (flag-free) variant of the node body
+  // next to the two parities would take the kernel from 67 to 113 VGPRs.
+  if (((b.parity0 + j) & 1) == 0)
+    node_update<K, 0, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
+  else
+    node_update<K, 1, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
+}
+
+// Tile order of the batched kernel: run >= -1 as tile_of (virtual tile
+// v = block + k * grid, one box search per tile); kRunBlockRanges: block b
+// takes the contiguous range [b * per, (b + 1) * per) of the packed list,
+// searches once and walks across box boundaries, skipping a box with no
+// step at j as a whole.
+constexpr int kRunBlockRanges = -2;
+
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
   const unsigned nbox = a.hdr[0], total = a.hdr[1];
-  const unsigned* start = a.hdr + 2;
+  const unsigned* start = a.hdr + 2;  // start[nbox] == total
+  if (a.run == kRunBlockRanges) {
+    const unsigned per = (total + gridDim.x - 1) / gridDim.x;
+    unsigned t = blockIdx.x * per;
+    const unsigned end = min(total, t + per);
+    if (t >= end) return;
+    unsigned lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+      const unsigned mid = (lo + hi) >> 1;
+      if (start[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    while (t < end) {
+      while (start[lo + 1] <= t) ++lo;
+      const BoxDesc& b = a.desc[lo];
+      const unsigned stop = min(end, start[lo + 1]);
+      if (a.j < b.steps)
+        for (; t < stop; ++t) batched_tile<K>(b, t - start[lo], a.j);
+      t = stop;
+    }
+    return;
+  }
   for (unsigned v = blockIdx.x; v < total; v += gridDim.x) {
     const unsigned t = tile_of(v, total, a.run);
     unsigned lo = 0, hi = nbox;  // largest lo with start[lo] <= t
@@ -302,34 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
     }
     const BoxDesc& b = a.desc[lo];
     if (a.j >= b.steps) continue;
-    const unsigned local = t - start[lo];
-    int x, y, z;
-    if (b.shape == kShapeLinear) {
-      const unsigned i = local * unsigned(kBlock) + threadIdx.x;
-      if (i >= b.nodes) continue;
-      const unsigned zz = fd_div(b.div_xy, i);
-      const unsigned r = i - zz * (unsigned(b.d.nx) * unsigned(b.d.ny));
-      const unsigned yy = fd_div(b.div_x, r);
-      x = int(r - yy * unsigned(b.d.nx));
-      y = int(yy);
-      z = int(zz);
-    } else {
-      int row;
-      tile_node(local, b.chunks, b.bx_log2, row, x);
-      if (row >= b.rows || x >= b.d.nx) continue;
-      z = int(fd_div(b.div_y, unsigned(row)));
-      y = row - z * b.d.ny;
-    }
-    Params<R> p;
-    p.omega = R(b.omega); p.omega_m = R(b.omega_m); p.omega_b = R(b.omega_b);
-    p.F[0] = R(b.F[0]); p.F[1] = R(b.F[1]); p.F[2] = R(b.F[2]);
-    T* A = static_cast<T*>(b.f);
-    // Flags are always checked: a second (flag-free) variant of the node body
-    // next to the two parities would take the kernel from 67 to 113 VGPRs.
-    if (((b.parity0 + a.j) & 1) == 0)
-      node_update<K, 0, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
-    else
-      node_update<K, 1, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
+    batched_tile<K>(b, t - start[lo], a.j);
   }
 }
 
@@ -803,12 +836,13 @@ __global__ void __launch_bounds__(kBlock) k_rescale(void* f, uint64_t stride, in
                                                     Ratio ratio) {
   using T = typename K::T;
   constexpr int Q = K::Q;
+  using C = typename K::C;
   T* A = static_cast<T*>(f);
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < np; n += gridDim.x * kBlock) {
     sfor<Q>([&](auto I) {
       constexpr int q = I;
       T& c = A[uint64_t(q) * stride + unsigned(n)];
-      c = T(float(c) * ratio.r[q]);
+      c = C::enc(float(c) * ratio.r[q], 1.0f);
     });
   }
 }

@@ -113,6 +113,21 @@ struct Codec<M3LB_BF16> {
   __device__ __forceinline__ static R dec(T v, float inv) { return float(v) * inv; }
   __device__ __forceinline__ static T enc(R v, float s) { return T(v * s); }
 };
+// 16-bit fixed point: code = round-to-nearest-even(s_q * g_q), saturated to
+// [-32767, 32767].  The error of a value is uniform, at most 2^-15 of the
+// pair's largest |g| when that maps into (2^13, 2^14], where FP16's error
+// grows with the value itself (2^-11 relative).
+template <>
+struct Codec<M3LB_I16> {
+  using T = int16_t;
+  using R = float;
+  static constexpr bool scaled = true;
+  __device__ __forceinline__ static R dec(T v, float inv) { return float(v) * inv; }
+  __device__ __forceinline__ static T enc(R v, float s) {
+    const float r = __builtin_rintf(v * s);
+    return T(__builtin_fminf(__builtin_fmaxf(r, -32767.0f), 32767.0f));
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Kernel configuration: everything the reference makes a template parameter

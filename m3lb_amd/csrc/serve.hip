@@ -112,6 +112,7 @@ struct m3lb_server {
   std::vector<int64_t> resident;  // admission order
   std::vector<int> free_slots;
   std::vector<int64_t> completed;  // not yet polled
+  uint64_t resident_nodes = 0;
   int64_t next_id = 1;
   // per buffer: descriptors and {boxes, tiles, tile_start[]} (pinned + device)
   BoxDesc* desc_h[kDepth] = {nullptr, nullptr};
@@ -154,6 +155,7 @@ int validate(const m3lb_server_config* c) {
   if (c->tile_shape < M3LB_SHAPE_ROWS || c->tile_shape > M3LB_SHAPE_AUTO)
     return err(M3LB_EINVAL, "unknown tile_shape");
   if (c->max_bypass < 0) return err(M3LB_EINVAL, "max_bypass must be >= 0");
+  if (c->tile_run < M3LB_TILE_BLOCK_RANGES) return err(M3LB_EINVAL, "unknown tile_run");
   if (c->budget_bytes < m3lb::Arena::kAlign) return err(M3LB_EINVAL, "budget_bytes too small");
   return M3LB_OK;
 }
@@ -299,13 +301,19 @@ int seed_box(m3lb_server* s, Box& b) {
 int admit(m3lb_server* s) {
   if (s->adm.size() == 0 || s->free_slots.empty()) return M3LB_OK;
   int rc = M3LB_OK;
+  const uint64_t cap = s->cfg.max_resident_nodes;
+  uint64_t nodes = s->resident_nodes;
   std::vector<int64_t> ids = s->adm.run(s->free_slots.size(), [&](int64_t id) {
     Box& b = s->boxes.at(id);
+    const uint64_t bn = uint64_t(b.d.nx) * uint64_t(b.d.ny) * uint64_t(b.d.nz);
+    if (cap && nodes > 0 && nodes + bn > cap) return false;
     const uint64_t off = s->arena.alloc(b.bytes);
     if (off == m3lb::Arena::kNone) return false;
     b.off = off;
+    nodes += bn;
     return true;
   });
+  s->resident_nodes = nodes;
   const double t = now_s(s);
   for (int64_t id : ids) {
     Box& b = s->boxes.at(id);
@@ -432,7 +440,10 @@ void free_box_memory(m3lb_server* s, Box& b) {
 
 void drop_resident(m3lb_server* s, int64_t id) {
   auto it = std::find(s->resident.begin(), s->resident.end(), id);
-  if (it != s->resident.end()) s->resident.erase(it);
+  if (it == s->resident.end()) return;
+  s->resident.erase(it);
+  const m3lb::Dom& d = s->boxes.at(id).d;
+  s->resident_nodes -= uint64_t(d.nx) * uint64_t(d.ny) * uint64_t(d.nz);
 }
 
 // Retires the oldest chunk if it has finished (or wait: once it finishes).
@@ -811,6 +822,7 @@ int m3lb_srv_stats(m3lb_server* s, m3lb_server_stats* out) {
   s->st.tile_run = s->cfg.tile_run;
   s->st.tile_shape = s->cfg.tile_shape;
   s->st.steps_per_chunk = s->cfg.steps_per_chunk;
+  s->st.resident_nodes = s->resident_nodes;
   *out = s->st;
   return M3LB_OK;
 }
@@ -911,7 +923,7 @@ int m3lb_srv_autotune(m3lb_server* s, int32_t steps, m3lb_tune_result* out) {
   int candidates = 0;
   if (!rc) rc = measure(&initial);
   static const int shapes[] = {M3LB_SHAPE_ROWS, M3LB_SHAPE_LINEAR, M3LB_SHAPE_AUTO};
-  static const int runs[] = {0, 8, 64};
+  static const int runs[] = {0, 8, 64, M3LB_TILE_BLOCK_RANGES};
   static const int bpcu[] = {4, 8, 16};
   for (int sh : shapes)
     for (int run : runs)

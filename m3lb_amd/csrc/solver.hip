@@ -60,8 +60,8 @@ int validate(const m3lb_config* c) {
   if (c->op < M3LB_OP_NULL || c->op > M3LB_OP_CM) return fail(M3LB_EINVAL, "unknown operator");
   if (c->storage != M3LB_RAW && c->storage != M3LB_SHIFTED)
     return fail(M3LB_EINVAL, "storage must be M3LB_RAW or M3LB_SHIFTED");
-  if (c->dtype < M3LB_F64 || c->dtype > M3LB_BF16) return fail(M3LB_EINVAL, "unknown dtype");
-  if ((c->dtype == M3LB_F16 || c->dtype == M3LB_BF16) &&
+  if (c->dtype < M3LB_F64 || c->dtype > M3LB_I16) return fail(M3LB_EINVAL, "unknown dtype");
+  if ((c->dtype == M3LB_F16 || c->dtype == M3LB_BF16 || c->dtype == M3LB_I16) &&
       (c->storage != M3LB_SHIFTED || c->op == M3LB_OP_NULL))
     return fail(M3LB_EINVAL, "16-bit storage holds shifted populations (g = f - w) only");
   if (c->forcing != M3LB_FORCE_NONE && c->forcing != M3LB_FORCE_GUO)
@@ -166,20 +166,48 @@ m3lb::SeedArgs seed_args(const m3lb_solver* s) {
 }
 
 bool is16(const m3lb_solver* s) {
-  return s->cfg.dtype == M3LB_F16 || s->cfg.dtype == M3LB_BF16;
+  return s->cfg.dtype == M3LB_F16 || s->cfg.dtype == M3LB_BF16 || s->cfg.dtype == M3LB_I16;
 }
 
-// Provisional 16-bit scales before the first seeding: 2^k with |g_q| <= w_q/2
-// mapped below 2^14 (FP16's largest finite code is 65504).
-void provisional_scales(m3lb_solver* s) {
+// 16-bit scales.  A scale is a power of two 2^k per opposite pair.
+//   F16, BF16: the pair's largest |g| maps into (2^7, 2^8]: FP16 then has 2^8
+//     of headroom under its largest finite code (65504) for the flow to grow
+//     into, and keeps full precision for values down to 2^-22 of that maximum
+//     (its smallest normal is 2^-14).  A floating format's relative precision
+//     is the same at every scale, so nothing is lost by mapping low.
+//   I16: the largest |g| maps into (2^13, 2^14]: 2x to 4x of headroom under
+//     the saturation at 32767, and a uniform error of at most 2^-15 of it.
+// A pair whose largest |g| is below 1e-3 w_q (a field at rest) is scaled as
+// if it were 1e-3 w_q.
+constexpr double kScaleFloor = 1e-3;
+
+double scale_target(int dtype) { return dtype == M3LB_I16 ? 16384.0 : 256.0; }
+
+double weight_of(int Q, int q) {
   static const int wn9[9] = {16, 4, 4, 4, 4, 1, 1, 1, 1};
   static const int wn27[27] = {64, 16, 16, 16, 16, 16, 16, 4, 4, 4, 4, 4, 4, 4,
                                4,  4,  4,  4,  4,  1,  1,  1, 1, 1, 1, 1, 1};
+  return Q == 9 ? wn9[q] / 36.0 : wn27[q] / 216.0;
+}
+
+// floor(log2(x)) exactly, for finite x > 0
+int floor_log2(double x) {
+  int e = 0;
+  std::frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+  return e - 1;
+}
+
+int scale_exponent(int dtype, int Q, int q, double max_abs) {
+  const double m = std::max(max_abs, kScaleFloor * weight_of(Q, q));
+  return std::max(-60, std::min(60, floor_log2(scale_target(dtype) / m)));
+}
+
+// Provisional 16-bit scales before the first seeding: |g_q| <= w_q/2 assumed.
+void provisional_scales(m3lb_solver* s) {
   for (int q = 0; q < 27; ++q) { s->scale[q] = 1.0f; s->inv_scale[q] = 1.0f; }
   if (!is16(s)) return;
   for (int q = 0; q < s->Q; ++q) {
-    const double w = s->Q == 9 ? wn9[q] / 36.0 : wn27[q] / 216.0;
-    const int k = int(std::floor(std::log2(16384.0 / (0.5 * w))));
+    const int k = scale_exponent(s->cfg.dtype, s->Q, q, 0.5 * weight_of(s->Q, q));
     s->scale[q] = std::ldexp(1.0f, k);
     s->inv_scale[q] = std::ldexp(1.0f, -k);
   }
@@ -804,8 +832,7 @@ int m3lb_calibrate_scales(m3lb_solver* s) {
 }  // extern "C"
 
 namespace {
-// Largest power-of-two scale per opposite pair that keeps max |g_q| * s_q
-// below 2^14 (a factor 4 of headroom under FP16's largest finite 65504).
+// Scales from the current field (see scale_exponent).
 int calibrate(m3lb_solver* s) {
   const unsigned grid = reduce_grid(s);
   int rc = ensure_partials(s, size_t(grid) * size_t(s->Q));
@@ -824,9 +851,7 @@ int calibrate(m3lb_solver* s) {
   for (int q = 0; q < s->Q; ++q) {
     const int o = q == 0 ? 0 : (q & 1 ? q + 1 : q - 1);
     const double m = std::max(mx[size_t(q)], mx[size_t(o)]);
-    int k = m > 0 ? int(std::floor(std::log2(16384.0 / m))) : 0;
-    k = std::max(-60, std::min(60, k));
-    want[size_t(q)] = std::ldexp(1.0, k);
+    want[size_t(q)] = std::ldexp(1.0, scale_exponent(s->cfg.dtype, s->Q, q, m));
   }
   return apply_scales(s, want.data());
 }

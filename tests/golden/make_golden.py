@@ -11,6 +11,8 @@ Sources (all in the reference checkout):
   results/E_tgv3d/tgv3d_re1600_d3q27_cm.dat        validation/tgv3d.cpp, D=64
   README.md, Galilean table                        validation/galilean.cpp
   README.md, Poiseuille slip and L2 tables         validation/poiseuille.cpp
+  results/H_forcing_convergence/forcing_conv.dat   validation/forcing_transverse.cpp -conv,
+                                                   D2Q9 CM, both orientations
 """
 import json
 import os
@@ -68,6 +70,9 @@ def main():
 
     p = "results/E_tgv3d/tgv3d_re1600_d3q27_cm.dat"
     data["tgv3d_re1600_d3q27_cm"] = {"file": p, "header": header(p), "rows": rows(p)}
+
+    p = "results/H_forcing_convergence/forcing_conv.dat"
+    data["forcing_transverse"] = {"file": p, "header": header(p), "rows": rows(p)}
 
     with open(os.path.join(REF, "README.md")) as f:
         lines = f.read().splitlines()

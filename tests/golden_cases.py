@@ -116,12 +116,14 @@ def tgv3d_run(solver_cls, probes, dtype=abi.F64, **kw):
     return out
 
 
-def momentum_exchange_np(cfg, flags, body, gathered):
+def momentum_exchange_np(cfg, flags, body, gathered, magnitude=False):
     """validation/square_cylinder.cpp:340-350 over every Fluid node: for each
     link (n, i) whose neighbour n + c_i (periodic wrap, else clamped into the
     padded box) is in the body, F += 2 c_i f(n, opp(i)), f = the gathered
     (stored) value unshifted in the compute type (unshift<C>).  flags and body
-    are interior arrays, x fastest; gathered is (n_padded, Q)."""
+    are interior arrays, x fastest; gathered is (n_padded, Q).  magnitude=True
+    also returns the sum of |2 c_i f| per axis, the scale of the rounding
+    error of F when its link terms cancel."""
     from m3lb_amd import lattice
     d = abi.domain_of(cfg)
     nx, ny, nz, hx, hy, hz, sx, sy, sz, npad = d
@@ -143,6 +145,7 @@ def momentum_exchange_np(cfg, flags, body, gathered):
         return np.clip(v, 0, s - 1)
 
     F = np.zeros(3)
+    M = np.zeros(3)
     per = cfg.periodic
     for i in range(1, Q):
         mx = shift(xx + c[i, 0], hx, nx, sx, per[0])
@@ -158,4 +161,110 @@ def momentum_exchange_np(cfg, flags, body, gathered):
         f = f.astype(np.float64)
         for a in range(min(3, c.shape[1])):
             F[a] += np.sum(2.0 * c[i, a] * f)
-    return F
+            M[a] += np.sum(np.abs(2.0 * c[i, a] * f))
+    return (F, M) if magnitude else F
+
+
+def forcing_transverse(solver_cls, orient, tau, H, umax=0.02, cap=600000, dtype=abi.F64):
+    """validation/forcing_transverse.cpp:53-126: D2Q9 central moments, Guo force
+    along the channel, halfway bounce-back walls.  orient 0: walls y-normal,
+    force along x; orient 1: walls x-normal, force along y.  Steps in probes of
+    200 until the centre velocity changes by less than 1e-15 relative, then
+    returns (L2 against the exact parabola, wall slip 0.5 - lower root)."""
+    across, along = H + 2, 8
+    nx, ny = (across, along) if orient else (along, across)
+    nu = (tau - 0.5) / 3.0
+    G = 8.0 * 1.0 * nu * umax / (H * H)
+    force = (0.0, G, 0.0) if orient else (G, 0.0, 0.0)
+    cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, dtype, (nx, ny, 1),
+                          periodic=(orient == 0, orient == 1, True), omega=1.0 / tau,
+                          force=force)
+    fl = np.zeros((ny, nx), dtype=np.uint8)
+    if orient:
+        fl[:, 0] = fl[:, across - 1] = abi.SOLID
+    else:
+        fl[0, :] = fl[across - 1, :] = abi.SOLID
+    d = abi.domain_of(cfg)
+    sx, hx, hy = d[6], d[3], d[4]
+    pid = lambda x, y: (y + hy) * sx + x + hx
+    s = solver_cls(cfg)
+    try:
+        s.set_geometry(fl.reshape(-1))
+        s.initialize(1.0)
+        comp = 2 if orient else 1
+        prev = 0.0
+        for t in range(0, cap, 200):
+            s.step(200)
+            cur = s.compute_macroscopic()[comp][pid(nx // 2, ny // 2)]
+            if t > 0 and abs(cur - prev) < 1e-15 * abs(cur):
+                break
+            prev = cur
+        m = s.compute_macroscopic()
+    finally:
+        s.close()
+    v, rho = m[comp], m[0]
+    ids = [pid(j, ny // 2) if orient else pid(nx // 2, j) for j in range(1, H + 1)]
+    rho_mean = sum(rho[i] for i in ids) / H
+    A = G / (2.0 * rho_mean * nu)
+    num = den = 0.0
+    for j, i in zip(range(1, H + 1), ids):
+        ana = A * (j - 0.5) * (H + 0.5 - j)
+        num += (v[i] - ana) ** 2
+        den += ana * ana
+    lo, _ = cases.parabola_roots([float(j) for j in range(1, H + 1)], [v[i] for i in ids])
+    return math.sqrt(num / den), 0.5 - lo
+
+
+def forcing_3d_channel(solver_cls, wall, fdir, tau, H=16, umax=0.02, cap=4000000,
+                       dtype=abi.F64):
+    """validation/forcing_3d.cpp:93-150 (PART 1): D3Q27 central moments, walls
+    normal to axis `wall`, Guo force along axis `fdir`, periodic elsewhere,
+    8 nodes along the other axes.  Probes every 200 steps until the centre
+    velocity changes by less than 1e-15 relative; returns the L2 error against
+    u(s) = G / (2 rho nu) (s - 0.5)(H + 0.5 - s)."""
+    n = [8, 8, 8]
+    n[wall] = H + 2
+    nu = (tau - 0.5) / 3.0
+    G = 8.0 * nu * umax / (H * H)
+    F = [0.0, 0.0, 0.0]
+    F[fdir] = G
+    per = [True, True, True]
+    per[wall] = False
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dtype, tuple(n), periodic=per,
+                          omega=1.0 / tau, force=tuple(F))
+    z, y, x = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    c = (x, y, z)[wall]
+    flags = np.where((c == 0) | (c == n[wall] - 1), abi.SOLID, abi.FLUID).astype(np.uint8)
+    d = abi.domain_of(cfg)
+    hx, hy, hz, sx, sy = d[3], d[4], d[5], d[6], d[7]
+    pid = lambda p: ((p[2] + hz) * sy + (p[1] + hy)) * sx + p[0] + hx
+
+    def at(j):
+        p = [n[0] // 2, n[1] // 2, n[2] // 2]
+        p[wall] = j
+        return pid(p)
+
+    s = solver_cls(cfg)
+    try:
+        s.set_geometry(flags.reshape(-1))
+        s.initialize(1.0)
+        comp = 1 + fdir
+        prev = 0.0
+        for t in range(0, cap, 200):
+            s.step(200)
+            cur = s.compute_macroscopic()[comp][at(n[wall] // 2)]
+            if t > 0 and abs(cur - prev) < 1e-15 * abs(cur):
+                break
+            prev = cur
+        m = s.compute_macroscopic()
+    finally:
+        s.close()
+    ids = [at(j) for j in range(1, H + 1)]
+    rho_mean = sum(m[0][i] for i in ids) / H
+    A = G / (2.0 * rho_mean * nu)
+    num = den = 0.0
+    for j, i in zip(range(1, H + 1), ids):
+        ana = A * (j - 0.5) * (H + 0.5 - j)
+        num += (m[comp][i] - ana) ** 2
+        den += ana * ana
+    return math.sqrt(num / den)

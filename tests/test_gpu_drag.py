@@ -41,11 +41,13 @@ def test_drag_matches_restatement(lat, op, storage, dtype):
         for b in (body, None):
             F = np.array(s.momentum_exchange(b))
             bb = (flags == abi.SOLID) if b is None else b
-            own = momentum_exchange_np(cfg, flags, bb, g_dev)
+            own, mag = momentum_exchange_np(cfg, flags, bb, g_dev, magnitude=True)
             orc = momentum_exchange_np(cfg, flags, bb, g_orc)
-            scale = np.max(np.abs(orc))
+            # error scale: the summed magnitude of the link terms, which
+            # cancel to a small F for raw (unshifted) populations
+            scale = np.max(mag)
             assert scale > 0
-            assert np.max(np.abs(F - own)) <= 1e-12 * scale, (F, own)
+            assert np.max(np.abs(F - own)) <= 1e-13 * scale, (F, own)
             assert np.max(np.abs(F - orc)) <= TOL[dtype] * scale, (F, orc)
 
 

@@ -74,10 +74,13 @@ MIX_D3 = [
 ]
 
 
-@pytest.mark.parametrize("shape_policy", [abi.SHAPE_ROWS, abi.SHAPE_LINEAR, abi.SHAPE_AUTO])
-def test_boxes_match_single_solver_cm_f32(shape_policy):
+@pytest.mark.parametrize("shape_policy,run", [(abi.SHAPE_ROWS, 64), (abi.SHAPE_LINEAR, 8),
+                                              (abi.SHAPE_AUTO, 0),
+                                              (abi.SHAPE_ROWS, abi.TILE_BLOCK_RANGES),
+                                              (abi.SHAPE_LINEAR, abi.TILE_BLOCK_RANGES)])
+def test_boxes_match_single_solver_cm_f32(shape_policy, run):
     scfg = abi.make_server_config(budget_bytes=512 * MiB, tile_shape=shape_policy,
-                                  steps_per_chunk=4)
+                                  tile_run=run, steps_per_chunk=4)
     reqs = [mk() for _, mk in MIX_D3]
     got = run_server(scfg, reqs)
     for k, (name, mk) in enumerate(MIX_D3):
@@ -242,7 +245,7 @@ def test_autotune_then_serve():
     with Server(scfg) as srv:
         t = srv.autotune(steps=2)
         print("autotune", t, flush=True)
-        assert t["candidates"] == 27 and t["best_glups"] > 0 and t["initial_glups"] > 0
+        assert t["candidates"] == 36 and t["best_glups"] > 0 and t["initial_glups"] > 0
         st = srv.stats()
         assert st["tile_shape"] == t["tile_shape"] and st["tile_run"] == t["tile_run"]
         assert st["bytes_used"] == 0
@@ -281,3 +284,34 @@ def test_taylor_green_initializer_matches_host_field():
         h.initialize_field(*cases.taylor_green_box(cfg, 0.05))
         a, b = s.gather_populations(), h.gather_populations()
     assert np.max(np.abs(a - b)) <= 1e-15
+
+
+def test_resident_node_cap():
+    """With a cap of two boxes' nodes, two boxes run at a time; the rest wait
+    in arrival order, and each result is the one it has alone."""
+    mk = lambda: tg((30, 26, 22), 9)
+    nodes = 30 * 26 * 22
+    scfg = abi.make_server_config(budget_bytes=256 * MiB, steps_per_chunk=4,
+                                  max_resident_nodes=2 * nodes)
+    ref = run_server(abi.make_server_config(budget_bytes=256 * MiB), [mk()])[0]
+    with Server(scfg) as srv:
+        ids = []
+        for _ in range(5):
+            r = mk()
+            r.keep = 1
+            ids.append(srv.submit(r))
+        srv.progress(0)
+        st = srv.stats()
+        assert st["resident"] == 2 and st["queued"] == 3
+        assert st["resident_nodes"] == 2 * nodes
+        srv.drain()
+        done = srv.poll()
+        assert [r["id"] for r in sorted(done, key=lambda r: r["admitted"])] == ids
+        for rid in ids:
+            assert np.array_equal(srv.fetch_populations(rid), ref[0])
+        # a box larger than the cap is still admitted into an empty server
+        big = tg((40, 40, 40), 2)
+        big.keep = 1
+        rb = srv.submit(big)
+        srv.drain()
+        assert srv.result(rb)["state"] == abi.REQ_DONE

@@ -140,3 +140,37 @@ def test_tgv3d_first_probe():
     assert t == 0.5 and ok0 and ok1
     assert rel(e1 / e0, E) < PRINT_TOL
     assert rel(w1 / w0, W) < PRINT_TOL
+
+
+def _forcing_rows(H_max):
+    return [r for r in GOLD["forcing_transverse"]["rows"] if r[1] <= H_max]
+
+
+@pytest.mark.parametrize("row", _forcing_rows(16), ids=lambda r: f"tau{r[0]}-H{int(r[1])}")
+def test_forcing_transverse_golden(row):
+    """results/H_forcing_convergence/forcing_conv.dat (forcing_transverse.cpp
+    -conv): both orientations of the forced D2Q9 CM channel; at tau = 7/8 both
+    are exact to round-off."""
+    from golden_cases import forcing_transverse
+    tau, H = row[0], int(row[1])
+    for orient, (l2_ref, slip_ref) in ((0, (row[2], row[4])), (1, (row[5], row[7]))):
+        l2, slip = forcing_transverse(OracleSolver, orient, tau, H)
+        if abs(tau - 0.875) < 1e-12:
+            assert l2 < 1e-11 and l2_ref < 1e-11, (orient, l2, l2_ref)
+            assert abs(slip) < 1e-11, (orient, slip)
+        else:
+            assert rel(l2, l2_ref) < PRINT_TOL, (orient, l2, l2_ref)
+            assert rel(slip, slip_ref) < 1e-6, (orient, slip, slip_ref)
+
+
+@pytest.mark.parametrize("tau", [0.875, 1.2])
+def test_forcing_3d_channels_every_axis(tau):
+    """forcing_3d.cpp PART 1 (:230-255): the three cyclic wall/force
+    orientations agree to round-off, and at tau = 7/8 each is exact."""
+    from golden_cases import forcing_3d_channel
+    l2 = [forcing_3d_channel(OracleSolver, c, (c + 1) % 3, tau, H=8) for c in range(3)]
+    assert max(l2) - min(l2) <= 1e-12, l2
+    if abs(tau - 0.875) < 1e-12:
+        assert max(l2) < 1e-11, l2
+    else:
+        assert min(l2) > 1e-6  # away from the magic point the parabola is not exact

@@ -24,15 +24,16 @@ def test_planning_code(tmp_path):
 
 def test_struct_layouts():
     """The serving structs as a C compiler lays them out on x86-64."""
-    assert ctypes.sizeof(abi.ServerConfig) == 56
+    assert ctypes.sizeof(abi.ServerConfig) == 64
     assert abi.ServerConfig.budget_bytes.offset == 48
+    assert abi.ServerConfig.max_resident_nodes.offset == 56
     assert ctypes.sizeof(abi.Request) == 128
     assert abi.Request.omega.offset == 40
     assert abi.Request.force.offset == 64
     assert abi.Request.arrival.offset == 120
     assert ctypes.sizeof(abi.Result) == 72
     assert abi.Result.mass.offset == 40
-    assert ctypes.sizeof(abi.ServerStats) == 120
+    assert ctypes.sizeof(abi.ServerStats) == 128
     assert abi.ServerStats.device_seconds.offset == 96
     assert ctypes.sizeof(abi.TuneResult) == 32
 
@@ -50,7 +51,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    assert [int(v) for v in out] == [56, 128, 120, 72, 120, 96, 32]
+    assert [int(v) for v in out] == [64, 128, 120, 72, 128, 96, 32]
 
 
 def _cfg(**kw):
@@ -76,6 +77,7 @@ def test_footprint(shape, periodic, dtype):
     dict(op=abi.OP_NULL), dict(dtype=abi.F16), dict(dtype=abi.BF16), dict(max_boxes=0),
     dict(steps_per_chunk=0), dict(steps_per_chunk=2000), dict(blocks_per_cu=0),
     dict(tile_shape=7), dict(max_bypass=-1), dict(budget_bytes=0), dict(forcing=5),
+    dict(tile_run=-3),
 ])
 def test_invalid_server_configs_are_rejected(bad):
     L = solver.load()

@@ -66,8 +66,12 @@ enum { M3LB_RAW = 0, M3LB_SHIFTED = 1 };
  * (src/core/Types.hpp:16-20) and has no 16-bit form. */
 enum { M3LB_F64 = 0, M3LB_F32 = 1, M3LB_F16 = 2, M3LB_BF16 = 3, M3LB_I16 = 4 };
 
-/* Forcing policy: NoForcing or uniform Guo (src/forcing/Forcing.hpp:41-82). */
-enum { M3LB_FORCE_NONE = 0, M3LB_FORCE_GUO = 1 };
+/* Forcing policy: NoForcing, uniform Guo (src/forcing/Forcing.hpp:41-82), or
+ * Guo with a per-node force F(n) set by m3lb_set_force_field -- the form of
+ * any forcing policy with at(n, F) (e.g. SinForce, validation/forcing_3d.cpp:
+ * 64-91), whose F enters the velocity shift and the Guo terms exactly as the
+ * uniform force does (Forcing.hpp:489-496). */
+enum { M3LB_FORCE_NONE = 0, M3LB_FORCE_GUO = 1, M3LB_FORCE_FIELD = 2 };
 
 /* Cell types (src/boundary/Flags.hpp:20-23).  Solid is halfway bounce-back,
  * the identity on esoteric-pull storage (src/memory/EsotericPull.hpp:27-33). */
@@ -185,6 +189,12 @@ int m3lb_timestep(const m3lb_solver* s, uint64_t* t);
 
 /* collision().forcing = Guo{fx,fy,fz} and collision().omega* at run time. */
 int m3lb_set_force(m3lb_solver* s, double fx, double fy, double fz);
+/* A forcing policy with a per-node F(n) (at(n, F), e.g. SinForce,
+ * validation/forcing_3d.cpp:64-91): for handles created with
+ * M3LB_FORCE_FIELD, three padded arrays of n_padded values (only Fluid nodes
+ * read them), converted to the compute type.  Until set, F = 0.  M3LB_ESTATE
+ * for other handles. */
+int m3lb_set_force_field(m3lb_solver* s, const double* fx, const double* fy, const double* fz);
 int m3lb_set_omega(m3lb_solver* s, double omega, double omega_minus, double omega_bulk);
 
 /* Per-direction scales of 16-bit storage: the stored code is s_q * g_q.

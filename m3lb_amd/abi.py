@@ -27,6 +27,7 @@ I16 = 4
 
 FORCE_NONE = 0
 FORCE_GUO = 1
+FORCE_FIELD = 2
 
 FLUID = 0
 SOLID = 1
@@ -67,7 +68,8 @@ class Config(ctypes.Structure):
 def make_config(lattice=D3Q27, op=OP_CM, storage=SHIFTED, dtype=F64, shape=(8, 8, 8),
                 periodic=(True, True, True), omega=1.0, omega_minus=1.0,
                 omega_bulk=-1.0, force=None):
-    """Builds a Config; force=None means NoForcing, a 3-tuple means Guo."""
+    """Builds a Config; force=None means NoForcing, a 3-tuple means uniform
+    Guo, "field" a per-node Guo force (set_force_field)."""
     nx, ny, nz = shape
     c = Config()
     c.lattice = lattice
@@ -82,6 +84,9 @@ def make_config(lattice=D3Q27, op=OP_CM, storage=SHIFTED, dtype=F64, shape=(8, 8
     c.omega_bulk = float(omega_bulk)
     if force is None:
         c.forcing = FORCE_NONE
+        f = (0.0, 0.0, 0.0)
+    elif isinstance(force, str) and force == "field":
+        c.forcing = FORCE_FIELD
         f = (0.0, 0.0, 0.0)
     else:
         c.forcing = FORCE_GUO

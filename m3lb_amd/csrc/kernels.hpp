@@ -34,8 +34,22 @@ struct StepArgs {
   double omega, omega_m, omega_b;
   double F[3];
   void* macro[4];  // rho, ux, uy, uz (compute type), or null
+  const void* ff;  // per-node force, 3 planes of `stride` (compute type), FIELD only
   float scale[27], inv_scale[27];
 };
+
+// The force a node sees: the uniform one, or for a per-node force policy the
+// node's own F from three planes of `stride` values.
+template <class K>
+__device__ __forceinline__ void force_at(Params<typename K::R>& p, const void* ff,
+                                         uint64_t stride, unsigned n) {
+  if constexpr (K::FIELD) {
+    const typename K::R* F = static_cast<const typename K::R*>(ff);
+    p.F[0] = F[n];
+    p.F[1] = F[stride + n];
+    p.F[2] = F[2 * stride + n];
+  }
+}
 
 template <class K>
 __device__ __forceinline__ Params<typename K::R> params_of(const StepArgs& a) {
@@ -216,14 +230,16 @@ __device__ __forceinline__ void tile_node(unsigned tile, int chunks, int bx_log2
 template <class K, int P, bool GEOM>
 __device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint64_t stride,
                                             const uint8_t* flags, const Dom& d, int x, int y,
-                                            int z, const Params<typename K::R>& p,
+                                            int z, const Params<typename K::R>& p0,
                                             const float* scale, const float* inv_scale,
-                                            void* const* macro) {
+                                            void* const* macro, const void* ff) {
   using R = typename K::R;
   const unsigned n = unsigned(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + d.hx + x);
   if constexpr (GEOM) {
     if (flags[n] != M3LB_FLUID) return;
   }
+  Params<R> p = p0;
+  force_at<K>(p, ff, stride, n);
   const Offs o = offsets(d, x, y, z);
   R f[K::Q];
   load_node<K, P>(A, stride, n, o, inv_scale, f);
@@ -246,7 +262,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   if (row >= a.rows || x >= d.nx) return;
   const Params<typename K::R> p = params_of<K>(a);
   node_update<K, P, GEOM>(static_cast<typename K::T*>(a.f), a.stride, a.flags, d, x,
-                          row % d.ny, row / d.ny, p, a.scale, a.inv_scale, a.macro);
+                          row % d.ny, row / d.ny, p, a.scale, a.inv_scale, a.macro, a.ff);
 }
 
 // ---------------------------------------------------------------------------
@@ -315,9 +331,9 @@ __device__ __forceinline__ void batched_tile(const BoxDesc& b, unsigned local, i
   // Flags are always checked: a second (flag-free) variant of the node body
   // next to the two parities would take the kernel from 67 to 113 VGPRs.
   if (((b.parity0 + j) & 1) == 0)
-    node_update<K, 0, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
+    node_update<K, 0, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr, nullptr);
   else
-    node_update<K, 1, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr);
+    node_update<K, 1, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr, nullptr);
 }
 
 // Tile order of the batched kernel: run >= -1 as tile_of (virtual tile
@@ -327,11 +343,13 @@ __device__ __forceinline__ void batched_tile(const BoxDesc& b, unsigned local, i
 // step at j as a whole.
 constexpr int kRunBlockRanges = -2;
 
-template <class K>
+// RANGES: the kRunBlockRanges order (a separate instance, so that neither
+// order's loop raises the other's register count).
+template <class K, bool RANGES>
 __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
   const unsigned nbox = a.hdr[0], total = a.hdr[1];
   const unsigned* start = a.hdr + 2;  // start[nbox] == total
-  if (a.run == kRunBlockRanges) {
+  if constexpr (RANGES) {
     const unsigned per = (total + gridDim.x - 1) / gridDim.x;
     unsigned t = blockIdx.x * per;
     const unsigned end = min(total, t + per);
@@ -519,6 +537,7 @@ struct ReadArgs {
   double* out2[4];
   float inv_scale[27];
   double omega, omega_m, omega_b, F[3];
+  const void* ff;  // per-node force planes (FIELD only)
 };
 
 __device__ __forceinline__ int shift_axis(int v, int h, int n, int s, int per) {
@@ -610,6 +629,7 @@ __global__ void __launch_bounds__(kBlock) k_macro(ReadArgs a) {
     Params<R> p;
     p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
     p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
+    force_at<K>(p, a.ff, a.stride, unsigned(n));
     R f[K::Q];
     const T* A = static_cast<const T*>(a.f);
     if (a.parity == 0) load_node<K, 0>(A, a.stride, unsigned(n), o, a.inv_scale, f);
@@ -669,6 +689,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mass(ReadArgs a) {
     Params<R> p;
     p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
     p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
+    force_at<K>(p, a.ff, a.stride, unsigned(n));
     R f[K::Q];
     const T* A = static_cast<const T*>(a.f);
     if (a.parity == 0) load_node<K, 0>(A, a.stride, unsigned(n), o, a.inv_scale, f);

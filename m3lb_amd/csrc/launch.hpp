@@ -31,7 +31,7 @@ inline unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBl
 
 template <class K>
 struct Launch {
-  static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::C::scaled;
+  static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::C::scaled && !K::FIELD;
   static void step(const StepArgs& a, int parity, bool geom, unsigned grid, hipStream_t s) {
     if (parity == 0) {
       if (geom) k_step<K, 0, true><<<grid, kBlock, 0, s>>>(a);
@@ -70,7 +70,10 @@ struct Launch {
     k_momentum_exchange<K><<<grid, kBlock, 0, s>>>(a, body);
   }
   static void batched(const BatchArgs& a, unsigned grid, hipStream_t s) {
-    if constexpr (kBatched) k_step_batched<K><<<grid, kBlock, 0, s>>>(a);
+    if constexpr (kBatched) {
+      if (a.run == kRunBlockRanges) k_step_batched<K, true><<<grid, kBlock, 0, s>>>(a);
+      else k_step_batched<K, false><<<grid, kBlock, 0, s>>>(a);
+    }
   }
   static const KernelSet* table() {
     static const KernelSet t = {step,     seed,      gather, storage,

@@ -134,11 +134,13 @@ struct Codec<M3LB_I16> {
 // of FluidSolver / the collision policy (FluidSolver.hpp:34-35, BGK.hpp:169-170,
 // MomentCollision.hpp:65-66).
 // ---------------------------------------------------------------------------
-template <int Q_, int OP_, int ST_, int DT_, bool FORCE_>
+template <int Q_, int OP_, int ST_, int DT_, int FORCE_>
 struct Cfg {
   static constexpr int Q = Q_, D = Lat<Q_>::D, OP = OP_, DT = DT_;
   static constexpr bool SHIFTED = (OP_ != M3LB_OP_NULL) && ST_ == M3LB_SHIFTED;
-  static constexpr bool FORCE = FORCE_ && OP_ != M3LB_OP_NULL;
+  // FORCE_: M3LB_FORCE_NONE / _GUO (uniform F) / _FIELD (F read per node)
+  static constexpr bool FORCE = FORCE_ != M3LB_FORCE_NONE && OP_ != M3LB_OP_NULL;
+  static constexpr bool FIELD = FORCE_ == M3LB_FORCE_FIELD && OP_ != M3LB_OP_NULL;
   using C = Codec<DT_>;
   using T = typename C::T;
   using R = typename C::R;

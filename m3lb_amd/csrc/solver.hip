@@ -64,8 +64,8 @@ int validate(const m3lb_config* c) {
   if ((c->dtype == M3LB_F16 || c->dtype == M3LB_BF16 || c->dtype == M3LB_I16) &&
       (c->storage != M3LB_SHIFTED || c->op == M3LB_OP_NULL))
     return fail(M3LB_EINVAL, "16-bit storage holds shifted populations (g = f - w) only");
-  if (c->forcing != M3LB_FORCE_NONE && c->forcing != M3LB_FORCE_GUO)
-    return fail(M3LB_EINVAL, "forcing must be M3LB_FORCE_NONE or M3LB_FORCE_GUO");
+  if (c->forcing != M3LB_FORCE_NONE && c->forcing != M3LB_FORCE_GUO && c->forcing != M3LB_FORCE_FIELD)
+    return fail(M3LB_EINVAL, "forcing must be M3LB_FORCE_NONE, M3LB_FORCE_GUO or M3LB_FORCE_FIELD");
   if (c->reserved != 0) return fail(M3LB_EINVAL, "reserved must be 0");
   if (c->nx < 1 || c->ny < 1 || c->nz < 1) return fail(M3LB_EINVAL, "extents must be >= 1");
   if (c->lattice == M3LB_D2Q9 && c->nz != 1) return fail(M3LB_EINVAL, "D2Q9 needs nz == 1");
@@ -100,6 +100,7 @@ struct m3lb_solver {
   float inv_scale[27];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int remap = 64;  // step-kernel tile order (kernels.hpp tile_of)
+  void* ff_d = nullptr;  // M3LB_FORCE_FIELD: fx, fy, fz planes of `stride` (compute type)
 };
 
 namespace {
@@ -133,6 +134,7 @@ m3lb::StepArgs step_args(const m3lb_solver* s) {
   a.omega_m = s->cfg.omega_minus;
   a.omega_b = s->cfg.omega_bulk;
   for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
+  a.ff = s->ff_d;
   for (int q = 0; q < 27; ++q) { a.scale[q] = s->scale[q]; a.inv_scale[q] = s->inv_scale[q]; }
   return a;
 }
@@ -149,6 +151,7 @@ m3lb::ReadArgs read_args(const m3lb_solver* s) {
   a.omega_m = s->cfg.omega_minus;
   a.omega_b = s->cfg.omega_bulk;
   for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
+  a.ff = s->ff_d;
   for (int q = 0; q < 27; ++q) a.inv_scale[q] = s->inv_scale[q];
   return a;
 }
@@ -296,6 +299,14 @@ int m3lb_create(const m3lb_config* cfg, m3lb_solver** out) {
     return bail(fail(M3LB_EDEVICE, "hipMemsetAsync failed"));
   if (hipMalloc(&s->flags_d, size_t(np)) != hipSuccess)
     return bail(fail(M3LB_ENOMEM, "flags: hipMalloc failed"));
+  if (cfg->forcing == M3LB_FORCE_FIELD) {
+    // the per-node force starts at zero everywhere
+    const size_t ffb = 3 * s->stride * size_t(ks->real_bytes);
+    if (hipMalloc(&s->ff_d, ffb) != hipSuccess)
+      return bail(fail(M3LB_ENOMEM, "force field: hipMalloc failed"));
+    if (hipMemsetAsync(s->ff_d, 0, ffb, s->stream) != hipSuccess)
+      return bail(fail(M3LB_EDEVICE, "hipMemsetAsync failed"));
+  }
   // Interior Fluid, halo Excluded (FluidSolver.hpp:59-67).
   s->flags_h.assign(size_t(np), uint8_t(M3LB_EXCLUDED));
   for (int z = 0; z < s->d.nz; ++z)
@@ -314,6 +325,7 @@ int m3lb_destroy(m3lb_solver* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   if (s->f) hipFree(s->f);
   if (s->flags_d) hipFree(s->flags_d);
+  if (s->ff_d) hipFree(s->ff_d);
   for (void* p : s->macro_d)
     if (p) hipFree(p);
   if (s->partial_d) hipFree(s->partial_d);
@@ -750,6 +762,27 @@ int m3lb_storage(m3lb_solver* s, double* out) { return read_pops(s, out, false);
 int m3lb_timestep(const m3lb_solver* s, uint64_t* t) {
   if (!s || !t) return fail(M3LB_EINVAL, "null argument");
   *t = s->t;
+  return M3LB_OK;
+}
+
+int m3lb_set_force_field(m3lb_solver* s, const double* fx, const double* fy, const double* fz) {
+  if (!s || !fx || !fy || !fz) return fail(M3LB_EINVAL, "null argument");
+  if (s->cfg.forcing != M3LB_FORCE_FIELD || !s->ff_d)
+    return fail(M3LB_ESTATE, "the handle was not created with M3LB_FORCE_FIELD");
+  const size_t np = size_t(s->d.np), st = size_t(s->stride);
+  const double* src[3] = {fx, fy, fz};
+  HIP_OK(hipStreamSynchronize(s->stream));  // the previous field may still be read
+  if (s->ks->real_bytes == 8) {
+    std::vector<double> h(3 * st, 0.0);
+    for (int k = 0; k < 3; ++k) std::copy(src[k], src[k] + np, h.begin() + k * st);
+    HIP_OK(hipMemcpy(s->ff_d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h(3 * st, 0.0f);
+    for (int k = 0; k < 3; ++k)
+      for (size_t n = 0; n < np; ++n) h[k * st + n] = float(src[k][n]);
+    HIP_OK(hipMemcpy(s->ff_d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  s->macro_valid = false;
   return M3LB_OK;
 }
 

@@ -42,6 +42,7 @@ SIGNATURES = {
     "m3lb_storage": (ctypes.c_int, [_vp, _dp]),
     "m3lb_timestep": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "m3lb_set_force": (ctypes.c_int, [_vp] + [ctypes.c_double] * 3),
+    "m3lb_set_force_field": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "m3lb_set_omega": (ctypes.c_int, [_vp] + [ctypes.c_double] * 3),
     "m3lb_get_scales": (ctypes.c_int, [_vp, _dp]),
     "m3lb_set_scales": (ctypes.c_int, [_vp, _dp]),
@@ -251,6 +252,13 @@ class FluidSolver:
         t = ctypes.c_uint64()
         _check(self.L.m3lb_timestep(self.h, ctypes.byref(t)))
         return t.value
+
+    def set_force_field(self, fx, fy, fz):
+        """m3lb_set_force_field: per-node Guo force, padded arrays."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (fx, fy, fz)]
+        for a in arrs:
+            assert a.size == self.np
+        _check(self.L.m3lb_set_force_field(self.h, *[_d(a) for a in arrs]))
 
     def set_force(self, fx, fy, fz):
         _check(self.L.m3lb_set_force(self.h, fx, fy, fz))

@@ -51,6 +51,7 @@ def load():
         "orc_storage": (None, [vp, _dp]),
         "orc_timestep": (ctypes.c_uint64, [vp]),
         "orc_set_force": (None, [vp] + [ctypes.c_double] * 3),
+        "orc_set_force_field": (None, [vp, _dp, _dp, _dp]),
         "orc_time_steps": (ctypes.c_double, [vp, ctypes.c_int]),
         "orc_collide_node": (None, [_cfgp, _dp, _dp, _dp]),
         "orc_macroscopic_node": (None, [_cfgp, _dp, _dp]),
@@ -162,6 +163,12 @@ class OracleSolver:
 
     def set_force(self, fx, fy, fz):
         self.L.orc_set_force(self.h, fx, fy, fz)
+
+    def set_force_field(self, fx, fy, fz):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (fx, fy, fz)]
+        for a in arrs:
+            assert a.size == self.dom[9]
+        self.L.orc_set_force_field(self.h, *[a.ctypes.data_as(_dp) for a in arrs])
 
 
 def collide_node(cfg, f):

@@ -61,6 +61,9 @@ uint64_t orc_timestep(const orc_solver* s) {
 void orc_set_force(orc_solver* s, double fx, double fy, double fz) {
   DISPATCH(s, set_force, fx, fy, fz);
 }
+void orc_set_force_field(orc_solver* s, const double* fx, const double* fy, const double* fz) {
+  DISPATCH(s, set_force_field, fx, fy, fz);
+}
 double orc_time_steps(orc_solver* s, int nsteps) { return DISPATCH(s, time_steps, nsteps); }
 
 #define NODE(name, cfg, ...) \

@@ -48,6 +48,8 @@ void orc_seed_populations(orc_solver* s, const double* in);
 void orc_storage(orc_solver* s, double* out);
 uint64_t orc_timestep(const orc_solver* s);
 void orc_set_force(orc_solver* s, double fx, double fy, double fz);
+/* Per-node Guo force for M3LB_FORCE_FIELD configs: three padded planes. */
+void orc_set_force_field(orc_solver* s, const double* fx, const double* fy, const double* fz);
 /* Wall-clock seconds for nsteps steps after one untimed warm-up step,
  * timed as src/app/main.cpp:41-47 does. */
 double orc_time_steps(orc_solver* s, int nsteps);

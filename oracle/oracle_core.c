@@ -89,7 +89,7 @@ static Coll coll_of(const m3lb_config* c) {
   k.l = lat_of(c->lattice);
   k.op = c->op;
   k.shifted = c->storage == M3LB_SHIFTED;
-  k.forcing = c->forcing == M3LB_FORCE_GUO;
+  k.forcing = c->forcing == M3LB_FORCE_GUO || c->forcing == M3LB_FORCE_FIELD;
   k.omega = R(c->omega);
   k.omega_m = R(c->omega_minus);
   k.omega_b = R(c->omega_bulk);
@@ -550,7 +550,21 @@ typedef struct {
   int64_t n_active;
   uint64_t t;
   int nthreads;
+  REAL* ff; /* FORCE_FIELD: fx, fy, fz planes of n_padded values, or NULL */
 } S;
+
+/* The collision policy at node n: for a per-node force (a forcing policy
+ * with at(n, F), e.g. validation/forcing_3d.cpp:64-91) a copy carrying that
+ * node's F; Guo's terms then use it as they use the uniform one
+ * (Forcing.hpp:489-496). */
+static inline const Coll* coll_at(const S* s, int64_t n, Coll* tmp) {
+  if (!s->ff) return &s->c;
+  *tmp = s->c;
+  tmp->F[0] = s->ff[n];
+  tmp->F[1] = s->ff[s->d.np + n];
+  tmp->F[2] = s->ff[2 * s->d.np + n];
+  return tmp;
+}
 
 #define F(s, n, q) ((s)->f[(n) * (int64_t)(s)->Q + (q)])
 
@@ -607,6 +621,7 @@ void* CORE(create)(const m3lb_config* cfg, int streaming) {
   s->f = (REAL*)calloc((size_t)(s->d.np * s->Q), sizeof(REAL));
   if (streaming) s->f2 = (REAL*)calloc((size_t)(s->d.np * s->Q), sizeof(REAL));
   s->active = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->d.np > 0 ? s->d.np : 1));
+  if (cfg->forcing == M3LB_FORCE_FIELD) s->ff = (REAL*)calloc((size_t)(3 * s->d.np), sizeof(REAL));
   for (int64_t n = 0; n < s->d.np; ++n) {
     int64_t px, py, pz;
     coords(&s->d, n, &px, &py, &pz);
@@ -619,7 +634,7 @@ void* CORE(create)(const m3lb_config* cfg, int streaming) {
 void CORE(destroy)(void* p) {
   S* s = (S*)p;
   if (!s) return;
-  free(s->flags); free(s->f); free(s->f2); free(s->active);
+  free(s->flags); free(s->f); free(s->f2); free(s->active); free(s->ff);
   free(s);
 }
 
@@ -752,8 +767,10 @@ static void* step_range(void* arg) {
         if (P == 0) { f[i] = F(s, n, i); f[i + 1] = F(s, j[i], i + 1); }
         else        { f[i] = F(s, n, i + 1); f[i + 1] = F(s, j[i], i); }
       }
-      const Macro m = macroscopic(&s->c, f);
-      collide(&s->c, &s->pt, f, &m);
+      Coll cn;
+      const Coll* cp = coll_at(s, n, &cn);
+      const Macro m = macroscopic(cp, f);
+      collide(cp, &s->pt, f, &m);
       F(s, n, 0) = f[0];
       for (int i = 1; i < Q; i += 2) {
         if (P == 0) { F(s, j[i], i + 1) = f[i]; F(s, n, i) = f[i + 1]; }
@@ -772,8 +789,10 @@ static void* step_range(void* arg) {
         for (int i = 0; i < Q; ++i) dst[n * Q + i] = f[opp(i)];
         continue;
       }
-      const Macro m = macroscopic(&s->c, f);
-      collide(&s->c, &s->pt, f, &m);
+      Coll cn;
+      const Coll* cp = coll_at(s, n, &cn);
+      const Macro m = macroscopic(cp, f);
+      collide(cp, &s->pt, f, &m);
       for (int i = 0; i < Q; ++i) dst[n * Q + i] = f[i];
     }
   }
@@ -820,6 +839,17 @@ double CORE(time_steps)(void* p, int nsteps) {
 
 uint64_t CORE(timestep)(const void* p) { return ((const S*)p)->t; }
 
+void CORE(set_force_field)(void* p, const double* fx, const double* fy, const double* fz) {
+  S* s = (S*)p;
+  const int64_t np = s->d.np;
+  if (!s->ff) s->ff = (REAL*)calloc((size_t)(3 * np), sizeof(REAL));
+  for (int64_t n = 0; n < np; ++n) {
+    s->ff[n] = R(fx[n]);
+    s->ff[np + n] = R(fy[n]);
+    s->ff[2 * np + n] = R(fz[n]);
+  }
+}
+
 void CORE(set_force)(void* p, double fx, double fy, double fz) {
   S* s = (S*)p;
   s->c.F[0] = R(fx); s->c.F[1] = R(fy); s->c.F[2] = R(fz);
@@ -847,7 +877,8 @@ void CORE(compute_macroscopic)(void* p, double* rho, double* ux, double* uy, dou
     REAL r = R(0), a = R(0), b = R(0), c = R(0);
     if (s->flags[n] == M3LB_FLUID) {
       load_node(s, n, f);
-      const Macro m = macroscopic(&s->c, f);
+      Coll cn;
+      const Macro m = macroscopic(coll_at(s, n, &cn), f);
       r = density(&s->c, &m); a = m.ux; b = m.uy; c = m.uz;
     }
     if (rho) rho[n] = (double)r;
@@ -864,7 +895,8 @@ static double reduce(S* s, int axis) {
   for (int64_t n = 0; n < s->d.np; ++n) {
     if (s->flags[n] != M3LB_FLUID) continue;
     load_node(s, n, f);
-    const Macro m = macroscopic(&s->c, f);
+    Coll cn;
+    const Macro m = macroscopic(coll_at(s, n, &cn), f);
     const REAL r = density(&s->c, &m);
     sum += (double)((axis < 0) ? r : r * (axis == 0 ? m.ux : (axis == 1 ? m.uy : m.uz)));
   }

@@ -20,6 +20,7 @@ void core64_step(void* p, int nsteps);
 double core64_time_steps(void* p, int nsteps);
 uint64_t core64_timestep(const void* p);
 void core64_set_force(void* p, double fx, double fy, double fz);
+void core64_set_force_field(void* p, const double* fx, const double* fy, const double* fz);
 void core64_compute_macroscopic(void* p, double* rho, double* ux, double* uy, double* uz);
 double core64_total_mass(void* p);
 double core64_total_momentum(void* p, int axis);
@@ -46,6 +47,7 @@ void core32_step(void* p, int nsteps);
 double core32_time_steps(void* p, int nsteps);
 uint64_t core32_timestep(const void* p);
 void core32_set_force(void* p, double fx, double fy, double fz);
+void core32_set_force_field(void* p, const double* fx, const double* fy, const double* fz);
 void core32_compute_macroscopic(void* p, double* rho, double* ux, double* uy, double* uz);
 double core32_total_mass(void* p);
 double core32_total_momentum(void* p, int axis);

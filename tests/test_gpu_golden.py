@@ -127,3 +127,57 @@ def test_forcing_3d_channels_every_axis(tau):
         assert max(l2) < 1e-11, l2
     else:
         assert min(l2) > 1e-6
+
+
+def test_forcing_3d_sine_converges_second_order():
+    """forcing_3d.cpp PART 2 on the device, N = 16, 32, 64 at tau = 0.8 and 1.2."""
+    from golden_cases import forcing_3d_sine
+    for tau in (0.8, 1.2):
+        l2 = [forcing_3d_sine(FluidSolver, N, tau)[0] for N in (16, 32, 64)]
+        orders = [math.log(a / b) / math.log(2.0) for a, b in zip(l2, l2[1:])]
+        assert all(1.95 < o < 2.05 for o in orders), (tau, l2, orders)
+
+
+@pytest.mark.parametrize("dtype", [abi.F64, abi.F32])
+def test_force_field_matches_oracle(dtype):
+    """A random per-node force on a channel with walls: device against oracle."""
+    import numpy as np
+    from m3lb_amd import cases
+    from oracle.binding import OracleSolver
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dtype, (20, 18, 16),
+                          periodic=(1, 0, 1), omega=1.4, force="field")
+    rng = np.random.Generator(np.random.MT19937(5))
+    n = abi.domain_of(cfg)[9]
+    F = [rng.uniform(-2e-5, 2e-5, n) for _ in range(3)]
+    fl = cases.channel_flags((20, 18, 16))
+    with FluidSolver(cfg) as s:
+        o = OracleSolver(cfg)
+        for x in (s, o):
+            x.set_geometry(fl)
+            x.set_force_field(*F)
+            x.initialize_field(*cases.taylor_green_box(cfg, 0.03))
+            x.step(15)
+        _, ux, uy, uz = s.compute_macroscopic()
+        _, vx, vy, vz = o.compute_macroscopic()
+        ma, mb = s.total_momentum(0), o.total_momentum(0)
+    num = np.sum((ux - vx) ** 2 + (uy - vy) ** 2 + (uz - vz) ** 2)
+    den = np.sum(vx ** 2 + vy ** 2 + vz ** 2)
+    tol = 1e-12 if dtype == abi.F64 else 1e-5
+    assert (num / den) ** 0.5 <= tol
+    assert abs(ma - mb) <= tol * 100 * abs(mb) + 1e-12
+
+
+def test_constant_force_field_is_uniform_guo():
+    import numpy as np
+    from m3lb_amd import cases
+    F = (2e-5, -1e-5, 3e-6)
+    shape = (24, 20, 16)
+    a = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F64, shape, omega=1.3, force=F)
+    b = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F64, shape, omega=1.3,
+                        force="field")
+    with FluidSolver(a) as sa, FluidSolver(b) as sb:
+        sb.set_force_field(*(np.full(sb.np, v) for v in F))
+        for s in (sa, sb):
+            s.initialize_field(*cases.taylor_green_box(a, 0.03))
+            s.step(7)
+        assert np.array_equal(sa.gather_populations(), sb.gather_populations())

@@ -174,3 +174,32 @@ def test_forcing_3d_channels_every_axis(tau):
         assert max(l2) < 1e-11, l2
     else:
         assert min(l2) > 1e-6  # away from the magic point the parabola is not exact
+
+
+def test_forcing_3d_sine_converges_second_order():
+    """forcing_3d.cpp PART 2 (:256-270) on the oracle: L2 error of the exact
+    3D forced solution falls at second order, N = 8 -> 16 -> 32."""
+    from golden_cases import forcing_3d_sine
+    l2 = [forcing_3d_sine(OracleSolver, N, 0.8, threads=8)[0] for N in (8, 16, 32)]
+    orders = [math.log(a / b) / math.log(2.0) for a, b in zip(l2, l2[1:])]
+    # 1.90 from 8 to 16 (pre-asymptotic), 1.98 from 16 to 32
+    assert 1.85 < orders[0] < 2.1 and 1.95 < orders[1] < 2.05, (l2, orders)
+    assert l2[-1] < 1e-2
+
+
+def test_constant_force_field_is_uniform_guo():
+    """A per-node force that is the same everywhere is uniform Guo, bit for bit."""
+    import numpy as np
+    shape = (6, 5, 4)
+    F = (2e-5, -1e-5, 3e-6)
+    a = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F64, shape, omega=1.3, force=F)
+    b = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F64, shape, omega=1.3,
+                        force="field")
+    fa, fb = OracleSolver(a), OracleSolver(b)
+    n = fa.dom[9]
+    fb.set_force_field(*(np.full(n, v) for v in F))
+    for s in (fa, fb):
+        s.initialize_field(*cases.taylor_green_box(a, 0.03))
+        s.step(7)
+    assert np.array_equal(fa.gather_populations(), fb.gather_populations())
+    assert fa.total_momentum(0) == fb.total_momentum(0)

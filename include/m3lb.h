@@ -202,8 +202,9 @@ int m3lb_set_omega(m3lb_solver* s, double omega, double omega_minus, double omeg
 int m3lb_get_scales(m3lb_solver* s, double* scales_q);
 int m3lb_set_scales(m3lb_solver* s, const double* scales_q);
 /* Re-chooses the 16-bit scales from the current field: per opposite pair the
- * power of two that maps max |g_q| into [2^7, 2^8) (256x headroom under
- * FP16's largest finite value; a pair below 1e-3 w_q counts as 1e-3 w_q).
+ * power of two that maps max |g_q| into (2^7, 2^8] for F16/BF16 (256x
+ * headroom under FP16's largest finite value) and (2^11, 2^12] for I16 (8x
+ * headroom under saturation); a pair below 1e-3 w_q counts as 1e-3 w_q.
  * Runs after every initialize; callers may repeat it on a schedule.  No-op
  * for F32/F64. */
 int m3lb_calibrate_scales(m3lb_solver* s);

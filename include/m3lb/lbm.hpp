@@ -60,6 +60,17 @@ struct Guo {
   static constexpr bool active = true;
   double fx = 0, fy = 0, fz = 0;
 };
+// A forcing policy with a per-node force, as the reference's SinForce
+// (validation/forcing_3d.cpp:64-91): any type with
+//     static constexpr bool active = true, per_node = true;
+//     void at(Index n, double F[3]) const;      // n: padded node index
+// evaluated on the host for every node and uploaded (M3LB_FORCE_FIELD).  After
+// changing the policy's parameters, call FluidSolver::update_force_field().
+template <class T, class = void>
+struct is_per_node_force : std::false_type {};
+template <class T>
+struct is_per_node_force<T, std::void_t<decltype(T::per_node)>>
+    : std::integral_constant<bool, T::per_node> {};
 
 // ---- equilibria (Equilibrium.hpp:126-136) -----------------------------------
 template <class L>
@@ -225,10 +236,25 @@ class FluidSolver {
     c.nx = d.nx; c.ny = d.ny; c.nz = d.nz;
     for (int a = 0; a < 3; ++a) c.periodic[a] = d.periodic[a] ? 1 : 0;
     push_rates(c);
-    c.forcing = Collision::ForcingPolicy::active ? M3LB_FORCE_GUO : M3LB_FORCE_NONE;
+    c.forcing = kPerNode ? M3LB_FORCE_FIELD
+                         : (Collision::ForcingPolicy::active ? M3LB_FORCE_GUO : M3LB_FORCE_NONE);
     push_force(c);
     cfg_ = c;
     check(m3lb_create(&cfg_, &h_), "m3lb_create");
+    if constexpr (kPerNode) update_force_field();
+  }
+
+  // Re-evaluates a per-node forcing policy's at(n, F) on every padded node.
+  void update_force_field() {
+    static_assert(kPerNode, "the forcing policy has no per-node force");
+    const std::size_t np = std::size_t(dom_.n_padded);
+    std::vector<double> fx(np, 0.0), fy(np, 0.0), fz(np, 0.0);
+    for (Index n = 0; n < dom_.n_padded; ++n) {
+      double F[3] = {0, 0, 0};
+      coll_.forcing.at(n, F);
+      fx[std::size_t(n)] = F[0]; fy[std::size_t(n)] = F[1]; fz[std::size_t(n)] = F[2];
+    }
+    check(m3lb_set_force_field(h_, fx.data(), fy.data(), fz.data()), "set_force_field");
   }
   ~FluidSolver() { m3lb_destroy(h_); }
   FluidSolver(const FluidSolver&) = delete;
@@ -327,8 +353,9 @@ class FluidSolver {
     c.omega_minus = coll_.rate(1);
     c.omega_bulk = coll_.rate(2);
   }
+  static constexpr bool kPerNode = is_per_node_force<typename Collision::ForcingPolicy>::value;
   void push_force(m3lb_config& c) const {
-    if constexpr (Collision::ForcingPolicy::active) {
+    if constexpr (Collision::ForcingPolicy::active && !kPerNode) {
       c.force[0] = coll_.forcing.fx; c.force[1] = coll_.forcing.fy; c.force[2] = coll_.forcing.fz;
     } else {
       c.force[0] = c.force[1] = c.force[2] = 0.0;
@@ -342,7 +369,7 @@ class FluidSolver {
     push_force(c);
     if (c.omega != cfg_.omega || c.omega_minus != cfg_.omega_minus || c.omega_bulk != cfg_.omega_bulk)
       check(m3lb_set_omega(h_, c.omega, c.omega_minus, c.omega_bulk), "set_omega");
-    if constexpr (Collision::ForcingPolicy::active) {
+    if constexpr (Collision::ForcingPolicy::active && !kPerNode) {
       if (c.force[0] != cfg_.force[0] || c.force[1] != cfg_.force[1] || c.force[2] != cfg_.force[2])
         check(m3lb_set_force(h_, c.force[0], c.force[1], c.force[2]), "set_force");
     }

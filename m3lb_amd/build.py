@@ -34,9 +34,9 @@ def _run(cmd):
     return cmd
 
 
-def _sources():
+def _headers():
     return [os.path.join(CSRC, f) for f in os.listdir(CSRC)
-            if f.endswith((".hip", ".hpp", ".h"))] + [os.path.join(ROOT, "include", "m3lb.h")]
+            if f.endswith((".hpp", ".h"))] + [os.path.join(ROOT, "include", "m3lb.h")]
 
 
 def _stale(obj, deps):
@@ -49,20 +49,20 @@ def _stale(obj, deps):
 def build(force=True, jobs=None, verbose=False):
     """Compiles every unit (force=True) or only those older than a source."""
     os.makedirs(OBJ, exist_ok=True)
-    deps = _sources()
+    headers = _headers()
     jobs = jobs or min(16, os.cpu_count() or 4)
     cmds = []
     objs = []
     for q, op in UNITS:
         o = os.path.join(OBJ, f"inst_q{q}_op{op}.o")
         objs.append(o)
-        if force or _stale(o, deps):
+        if force or _stale(o, headers + [os.path.join(CSRC, "inst.hip")]):
             cmds.append([HIPCC] + FLAGS + [f"-DM3LB_Q={q}", f"-DM3LB_OP={op}", "-c",
                                            os.path.join(CSRC, "inst.hip"), "-o", o])
     for unit in ("solver", "serve"):
         o = os.path.join(OBJ, f"{unit}.o")
         objs.append(o)
-        if force or _stale(o, deps):
+        if force or _stale(o, headers + [os.path.join(CSRC, f"{unit}.hip")]):
             cmds.append([HIPCC] + FLAGS + ["-c", os.path.join(CSRC, f"{unit}.hip"), "-o", o])
     # largest units first
     cmds.sort(key=lambda c: ("q27" not in c[-1], c[-1]))

@@ -114,9 +114,8 @@ struct Codec<M3LB_BF16> {
   __device__ __forceinline__ static T enc(R v, float s) { return T(v * s); }
 };
 // 16-bit fixed point: code = round-to-nearest-even(s_q * g_q), saturated to
-// [-32767, 32767].  The error of a value is uniform, at most 2^-15 of the
-// pair's largest |g| when that maps into (2^13, 2^14], where FP16's error
-// grows with the value itself (2^-11 relative).
+// [-32767, 32767].  The error of a value is uniform (half a code), where
+// FP16's grows with the value itself (2^-11 relative).
 template <>
 struct Codec<M3LB_I16> {
   using T = int16_t;

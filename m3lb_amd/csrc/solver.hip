@@ -178,13 +178,14 @@ bool is16(const m3lb_solver* s) {
 //     into, and keeps full precision for values down to 2^-22 of that maximum
 //     (its smallest normal is 2^-14).  A floating format's relative precision
 //     is the same at every scale, so nothing is lost by mapping low.
-//   I16: the largest |g| maps into (2^13, 2^14]: 2x to 4x of headroom under
-//     the saturation at 32767, and a uniform error of at most 2^-15 of it.
+//   I16: scales proportional to the weights (see measure_scales), the
+//     largest |g| mapped into (2^11, 2^12]: 8x to 16x of headroom under the
+//     saturation at 32767, and a uniform error of half a code.
 // A pair whose largest |g| is below 1e-3 w_q (a field at rest) is scaled as
 // if it were 1e-3 w_q.
 constexpr double kScaleFloor = 1e-3;
 
-double scale_target(int dtype) { return dtype == M3LB_I16 ? 16384.0 : 256.0; }
+double scale_target(int dtype) { return dtype == M3LB_I16 ? 4096.0 : 256.0; }
 
 double weight_of(int Q, int q) {
   static const int wn9[9] = {16, 4, 4, 4, 4, 1, 1, 1, 1};
@@ -228,6 +229,7 @@ int sync(m3lb_solver* s) {
 }
 
 int calibrate(m3lb_solver* s);
+int seed_calibrated(m3lb_solver* s, m3lb::SeedArgs a);
 
 }  // namespace
 
@@ -363,12 +365,7 @@ int m3lb_initialize(m3lb_solver* s, double rho0, double ux, double uy, double uz
   a.mode = m3lb::kSeedUniform;
   a.rho0 = rho0;
   a.u0[0] = ux; a.u0[1] = uy; a.u0[2] = uz;
-  s->ks->seed(a, s->stream);
-  int rc = launch_ok();
-  if (rc) return rc;
-  rc = sync(s);
-  if (rc) return rc;
-  return is16(s) ? calibrate(s) : M3LB_OK;
+  return seed_calibrated(s, a);
 }
 
 int m3lb_initialize_taylor_green(m3lb_solver* s, double u0) {
@@ -380,12 +377,7 @@ int m3lb_initialize_taylor_green(m3lb_solver* s, double u0) {
   m3lb::SeedArgs a = seed_args(s);
   a.mode = m3lb::kSeedTaylorGreen;
   a.u0[0] = u0;
-  s->ks->seed(a, s->stream);
-  int rc = launch_ok();
-  if (rc) return rc;
-  rc = sync(s);
-  if (rc) return rc;
-  return is16(s) ? calibrate(s) : M3LB_OK;
+  return seed_calibrated(s, a);
 }
 
 int m3lb_initialize_field(m3lb_solver* s, const double* rho, const double* ux, const double* uy,
@@ -403,13 +395,9 @@ int m3lb_initialize_field(m3lb_solver* s, const double* rho, const double* ux, c
   m3lb::SeedArgs a = seed_args(s);
   a.mode = m3lb::kSeedField;
   for (int k = 0; k < 4; ++k) a.field[k] = buf + k * np;
-  s->ks->seed(a, s->stream);
-  int rc = launch_ok();
-  hipError_t e = hipStreamSynchronize(s->stream);
-  hipFree(buf);
-  if (rc) return rc;
-  if (e != hipSuccess) return fail(M3LB_EDEVICE, hipGetErrorString(e));
-  return is16(s) ? calibrate(s) : M3LB_OK;
+  const int rc = seed_calibrated(s, a);  // synchronises before returning
+  (void)hipFree(buf);
+  return rc;
 }
 
 int m3lb_seed_populations(m3lb_solver* s, const double* in) {
@@ -865,8 +853,8 @@ int m3lb_calibrate_scales(m3lb_solver* s) {
 }  // extern "C"
 
 namespace {
-// Scales from the current field (see scale_exponent).
-int calibrate(m3lb_solver* s) {
+// Per-pair scales the current field calls for (see scale_exponent).
+int measure_scales(m3lb_solver* s, std::vector<double>& want) {
   const unsigned grid = reduce_grid(s);
   int rc = ensure_partials(s, size_t(grid) * size_t(s->Q));
   if (rc) return rc;
@@ -878,14 +866,50 @@ int calibrate(m3lb_solver* s) {
   std::vector<double> h(size_t(grid) * size_t(s->Q));
   HIP_OK(hipMemcpyAsync(h.data(), s->partial_d, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipStreamSynchronize(s->stream));
-  std::vector<double> mx(size_t(s->Q), 0.0), want(size_t(s->Q));
+  std::vector<double> mx(size_t(s->Q), 0.0);
+  want.assign(size_t(s->Q), 1.0);
   for (unsigned b = 0; b < grid; ++b)
     for (int q = 0; q < s->Q; ++q) mx[size_t(q)] = std::max(mx[size_t(q)], h[size_t(b) * s->Q + q]);
+  // I16 scales every pair in proportion to its weight, from the largest
+  // |g_q| / w_q over all directions: a direction that starts near zero (e.g.
+  // along an axis the initial flow has no component on) would otherwise get a
+  // scale it saturates at once the flow turns into it.
+  double ratio = 0;
+  for (int q = 0; q < s->Q; ++q) ratio = std::max(ratio, mx[size_t(q)] / weight_of(s->Q, q));
   for (int q = 0; q < s->Q; ++q) {
     const int o = q == 0 ? 0 : (q & 1 ? q + 1 : q - 1);
-    const double m = std::max(mx[size_t(q)], mx[size_t(o)]);
+    const double m = s->cfg.dtype == M3LB_I16 ? ratio * weight_of(s->Q, q)
+                                              : std::max(mx[size_t(q)], mx[size_t(o)]);
     want[size_t(q)] = std::ldexp(1.0, scale_exponent(s->cfg.dtype, s->Q, q, m));
   }
-  return apply_scales(s, want.data());
+  return M3LB_OK;
+}
+
+// Scales from the current field; the codes are re-encoded in place.
+int calibrate(m3lb_solver* s) {
+  std::vector<double> want;
+  const int rc = measure_scales(s, want);
+  return rc ? rc : apply_scales(s, want.data());
+}
+
+// Seeds, and for 16-bit storage seeds a second time from the same source at
+// the scales the first seeding calls for, so no value keeps the rounding of
+// the provisional scale (which a rescale of fixed-point codes cannot undo).
+int seed_calibrated(m3lb_solver* s, m3lb::SeedArgs a) {
+  s->ks->seed(a, s->stream);
+  int rc = launch_ok();
+  if (!rc) rc = sync(s);
+  if (rc || !is16(s)) return rc;
+  std::vector<double> want;
+  rc = measure_scales(s, want);
+  if (rc) return rc;
+  for (int q = 0; q < s->Q; ++q) {
+    s->scale[q] = float(want[size_t(q)]);
+    s->inv_scale[q] = float(1.0 / want[size_t(q)]);
+    a.scale[q] = s->scale[q];
+  }
+  s->ks->seed(a, s->stream);
+  rc = launch_ok();
+  return rc ? rc : sync(s);
 }
 }  // namespace

@@ -39,3 +39,21 @@ def test_facade_poiseuille_matches_reference_table():
             else:
                 assert abs(slip - want) <= 0.0051 * abs(want), (key, slip, want)
     assert len(got) == 12
+
+
+def test_facade_per_node_force_policy():
+    """tests/cpp/sine_force_facade.cpp: a forcing policy written as the
+    reference's SinForce (validation/forcing_3d.cpp:64-91) through the facade
+    gives the same steady state as the same case driven from Python."""
+    from golden_cases import forcing_3d_sine
+    from m3lb_amd.solver import FluidSolver
+    exe = os.path.join(os.path.dirname(EXE), "sine_force_facade")
+    print("\n[gpu-case] facade sine force", flush=True)
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = [line.split() for line in r.stdout.split("\n") if line.strip()]
+    assert len(rows) == 4
+    for N, tau, l2, _ in rows:
+        want, _ = forcing_3d_sine(FluidSolver, int(N), float(tau))
+        assert abs(float(l2) - want) <= 1e-6 * want, (N, tau, l2, want)

@@ -1,0 +1,76 @@
+"""16-bit storage against an FP64-storage run of the same Taylor-Green case
+(tgv3d_bench field, Re = 1600 on L, seeded 1e-4 perturbation) on the GPU.
+Velocity relative L2 after a fixed number of steps, with the tolerance each
+form is held to written next to it, and the scale invariants."""
+import math
+
+import numpy as np
+import pytest
+
+from m3lb_amd import abi, cases
+from m3lb_amd.solver import FluidSolver
+
+pytestmark = pytest.mark.gpu
+
+D, STEPS, U0 = 64, 300, 0.04
+
+
+def _run(dtype, field, omega):
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dtype, (D, D, D), omega=omega)
+    with FluidSolver(cfg) as s:
+        s.initialize_field(*field)
+        sc = s.scales()
+        s.step(STEPS)
+        _, ux, uy, uz = s.compute_macroscopic()
+        e, _, fin = s.energy_enstrophy()
+        st = s.storage() if dtype in (abi.F16, abi.BF16, abi.I16) else None
+    return (ux, uy, uz), e, fin, sc, st
+
+
+@pytest.fixture(scope="module")
+def runs():
+    nu = U0 * (D / (2 * math.pi)) / 1600.0
+    omega = 1.0 / (3 * nu + 0.5)
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F64, (D, D, D), omega=omega)
+    field = cases.tgv3d_benchmark(cfg, U0, perturbation=1e-4, seed=3)
+    return {dt: _run(dt, field, omega) for dt in (abi.F64, abi.F32, abi.F16, abi.BF16, abi.I16)}
+
+
+def _rel(a, b):
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
+    den = sum(float(np.sum(y ** 2)) for y in b)
+    return math.sqrt(num / den)
+
+
+# velocity relative L2 against FP64 storage after 300 steps at 64^3
+TOL = {abi.F32: 1e-5, abi.F16: 1e-3, abi.BF16: 5e-2, abi.I16: 1e-3}
+
+
+@pytest.mark.parametrize("dtype", [abi.F32, abi.F16, abi.BF16, abi.I16])
+def test_velocity_against_fp64_storage(runs, dtype):
+    ref = runs[abi.F64]
+    u, e, fin, _, _ = runs[dtype]
+    assert fin
+    err = _rel(u, ref[0])
+    print(f"{abi.DTYPE_NAMES[dtype]}: velocity rel-L2 {err:.3e}, dE {abs(e - ref[1]) / ref[1]:.3e}")
+    assert err <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [abi.F16, abi.BF16, abi.I16])
+def test_scales_are_shared_powers_of_two(runs, dtype):
+    sc = runs[dtype][3]
+    for q in range(27):
+        m, _ = math.frexp(sc[q])
+        assert m == 0.5
+        o = 0 if q == 0 else (q + 1 if q & 1 else q - 1)
+        assert sc[q] == sc[o]
+
+
+@pytest.mark.parametrize("dtype", [abi.F16, abi.I16])
+def test_codes_keep_headroom(runs, dtype):
+    """After the run, the largest stored code stays below the format's limit
+    with room to spare (FP16: 65504, I16: saturation at 32767)."""
+    _, _, _, sc, st = runs[dtype]
+    codes = np.abs(st * np.asarray(sc)[None, :])
+    limit = 65504.0 if dtype == abi.F16 else 32767.0
+    assert codes.max() < 0.5 * limit

@@ -76,6 +76,28 @@ enum { M3LB_FORCE_NONE = 0, M3LB_FORCE_GUO = 1, M3LB_FORCE_FIELD = 2 };
 /* Cell types (src/boundary/Flags.hpp:20-23).  Solid is halfway bounce-back,
  * the identity on esoteric-pull storage (src/memory/EsotericPull.hpp:27-33). */
 enum { M3LB_FLUID = 0, M3LB_SOLID = 1, M3LB_EXCLUDED = 2 };
+/* Boundary node types set only by their own calls, never by
+ * m3lb_set_geometry (src/boundary/Flags.hpp:24-33): a regularised wall node
+ * (m3lb_set_regularized_walls), a ghost specular cell (m3lb_set_specular_walls:
+ * permutes its incoming set about the wall normal, no collision, no
+ * macroscopic state) and an on-node specular node (m3lb_set_specular_nodes:
+ * mirrors its unknown directions, then collides as fluid). */
+enum { M3LB_REGWALL = 3, M3LB_SPECWALL = 4, M3LB_SPECNODE = 5 };
+
+/* Wall normal codes of m3lb_set_regularized_walls and m3lb_set_specular_walls
+ * (boundary/Regularized.hpp:97-142): outward normals, a corner, and the three
+ * outflow closures (rho imposed from the table, or for OUT_FREE from
+ * m3lb_set_outflow_density). */
+enum {
+  M3LB_NRM_NONE = 0, M3LB_NRM_XP = 1, M3LB_NRM_XM = 2, M3LB_NRM_YP = 3, M3LB_NRM_YM = 4,
+  M3LB_NRM_ZP = 5, M3LB_NRM_ZM = 6, M3LB_NRM_CORNER = 7, M3LB_NRM_OUT_XP = 8,
+  M3LB_NRM_OUT_FREE = 9, M3LB_NRM_OUT_EQ = 10
+};
+/* Face mask bits of m3lb_set_specular_nodes (Specular.hpp:241-249). */
+enum {
+  M3LB_SPEC_XP = 1, M3LB_SPEC_XM = 2, M3LB_SPEC_YP = 4, M3LB_SPEC_YM = 8,
+  M3LB_SPEC_ZP = 16, M3LB_SPEC_ZM = 32
+};
 
 /* Status codes. */
 enum {
@@ -189,6 +211,23 @@ int m3lb_timestep(const m3lb_solver* s, uint64_t* t);
 
 /* collision().forcing = Guo{fx,fy,fz} and collision().omega* at run time. */
 int m3lb_set_force(m3lb_solver* s, double fx, double fy, double fz);
+/* set_regularized_walls(fn -> WallSpec) (FluidSolver.hpp:84-99, 372-460):
+ * regularised velocity walls (Latt et al. 2008) on the nodes themselves.
+ * normals: one code per INTERIOR node, x fastest -- 0 none, 1..6 the OUTWARD
+ * normal +x, -x, +y, -y, +z, -z, 7 a corner (where two walls meet: density
+ * extrapolated along a wall, stress from finite-difference gradients unless
+ * m3lb_set_fd_corners(s, 0)); tags: index into table (the reference's
+ * deduplicated (ux, uy, uz, rho) table).  The nodes become M3LB_REGWALL.
+ * Unknown directions are fixed from the flags as they are at this call;
+ * m3lb_set_geometry afterwards removes the walls.  Straight walls and
+ * corners only: the outflow codes and specular walls are not provided. */
+typedef struct m3lb_wall {
+  double ux, uy, uz, rho;
+} m3lb_wall;
+int m3lb_set_regularized_walls(m3lb_solver* s, const uint8_t* normals, const uint16_t* tags,
+                                const m3lb_wall* table, int32_t ntable);
+/* set_fd_corners(on) (FluidSolver.hpp:272); on by default. */
+int m3lb_set_fd_corners(m3lb_solver* s, int32_t on);
 /* A forcing policy with a per-node F(n) (at(n, F), e.g. SinForce,
  * validation/forcing_3d.cpp:64-91): for handles created with
  * M3LB_FORCE_FIELD, three padded arrays of n_padded values (only Fluid nodes

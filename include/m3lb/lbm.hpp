@@ -158,7 +158,12 @@ template <class L, class F = NoForcing, class S = RawPopulations>
 using MRT = MomentCollision<L, F, S, false>;
 
 // ---- cells and domain (Flags.hpp:20-23, Domain.hpp:37-124) ------------------
-enum CellType : std::uint8_t { Fluid = 0, Solid = 1, Excluded = 2 };
+enum CellType : std::uint8_t { Fluid = 0, Solid = 1, Excluded = 2, RegWall = 3 };
+
+// wall normal codes, pointing out of the fluid (boundary/Regularized.hpp:98-103)
+enum WallNormal : std::uint8_t {
+  NrmNone = 0, NrmXp = 1, NrmXm = 2, NrmYp = 3, NrmYm = 4, NrmZp = 5, NrmZm = 6, NrmCorner = 7
+};
 
 struct FlowState {  // src/solver/FluidSolver.hpp:30-32
   double rho = 1.0, ux = 0.0, uy = 0.0, uz = 0.0;
@@ -270,6 +275,44 @@ class FluidSolver {
         for (Index x = 0; x < dom_.nx; ++x) fl[k++] = static_cast<std::uint8_t>(fn(x, y, z));
     check(m3lb_set_geometry(h_, fl.data()), "set_geometry");
   }
+
+  // set_regularized_walls(fn(x, y, z) -> WallSpec), FluidSolver.hpp:84-99,
+  // 372-460: velocity walls on the nodes themselves; the velocities are
+  // deduplicated into a table.  Call after set_geometry.
+  struct WallSpec {
+    std::uint8_t normal = NrmNone;
+    double ux = 0, uy = 0, uz = 0;
+  };
+  template <class Fn>
+  void set_regularized_walls(Fn fn) {
+    const std::size_t ni = std::size_t(dom_.n_interior());
+    std::vector<std::uint8_t> nrm(ni, 0);
+    std::vector<std::uint16_t> tag(ni, 0);
+    std::vector<m3lb_wall> table;
+    std::size_t k = 0;
+    for (Index z = 0; z < dom_.nz; ++z)
+      for (Index y = 0; y < dom_.ny; ++y)
+        for (Index x = 0; x < dom_.nx; ++x, ++k) {
+          const WallSpec w = fn(x, y, z);
+          if (w.normal == NrmNone) continue;
+          std::size_t t = 0;
+          while (t < table.size() &&
+                 !(table[t].ux == w.ux && table[t].uy == w.uy && table[t].uz == w.uz))
+            ++t;
+          if (t == table.size()) {
+            if (t == 65536) throw std::runtime_error("set_regularized_walls: over 65536 velocities");
+            table.push_back(m3lb_wall{w.ux, w.uy, w.uz, 0.0});
+          }
+          nrm[k] = w.normal;
+          tag[k] = std::uint16_t(t);
+        }
+    if (table.empty()) table.push_back(m3lb_wall{0, 0, 0, 0});
+    check(m3lb_set_regularized_walls(h_, nrm.data(), tag.data(), table.data(),
+                                     std::int32_t(table.size())),
+          "set_regularized_walls");
+  }
+  // set_fd_corners(on), FluidSolver.hpp:272
+  void set_fd_corners(bool on) { check(m3lb_set_fd_corners(h_, on ? 1 : 0), "set_fd_corners"); }
 
   void initialize(double rho0, double ux0 = 0, double uy0 = 0, double uz0 = 0) {
     sync_policy();

@@ -331,3 +331,75 @@ def sphere_channel(shape, n_spheres=200, rmin=6.0, rmax=14.0, seed=2, max_tries=
     flags[nz - 1, :, :] = abi.SOLID
     flags[body] = abi.SOLID
     return flags.reshape(-1), body.reshape(-1).astype(np.uint8), spheres
+
+
+def shear_layer(cfg, u0, kappa=80.0, delta=0.05):
+    """validation/shear_layer.cpp:43-54: double shear layer on an L x L
+    periodic box, rho = 1, ux = u0 tanh(kappa (y/L - 1/4)) for y/L <= 1/2 and
+    u0 tanh(kappa (3/4 - y/L)) above, uy = u0 delta sin(2 pi (x/L + 1/4))."""
+    x, y, _, d = padded_coords(cfg)
+    L = cfg.nx
+    xx = (x - d[3]) / L
+    yy = (y - d[4]) / L
+    ux = np.where(yy <= 0.5, u0 * np.tanh(kappa * (yy - 0.25)), u0 * np.tanh(kappa * (0.75 - yy)))
+    uy = u0 * delta * np.sin(2.0 * math.pi * (xx + 0.25))
+    return np.ones_like(ux), ux, uy, np.zeros_like(ux)
+
+
+# Outward wall normals of regularised velocity walls (boundary/Regularized.hpp
+# NormalCode): 0 none, +x, -x, +y, -y, +z, -z, corner.
+NRM_NONE, NRM_XP, NRM_XM, NRM_YP, NRM_YM, NRM_ZP, NRM_ZM, NRM_CORNER = range(8)
+
+
+def cavity_walls(N, U, nz=1):
+    """validation/cavity_cm.cpp:93-101: an N x N (x nz) box whose edge nodes
+    are regularised walls -- the lid y = N-1 moving at (U, 0, 0), the other
+    sides at rest, the four corner columns marked corner.  Returns interior
+    (normals, tags) x fastest and the velocity table (ux, uy, uz, rho)."""
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(N), np.arange(N), indexing="ij")
+    xl, xr, yb, yt = x == 0, x == N - 1, y == 0, y == N - 1
+    nrm = np.zeros(x.shape, dtype=np.uint8)
+    tag = np.zeros(x.shape, dtype=np.uint16)
+    nrm[xr] = NRM_XP
+    nrm[xl] = NRM_XM
+    nrm[yb] = NRM_YM
+    nrm[yt] = NRM_YP
+    tag[yt] = 1
+    nrm[(xl | xr) & (yb | yt)] = NRM_CORNER
+    tag[(xl | xr) & (yb | yt)] = 0
+    table = np.array([[0.0, 0.0, 0.0, 0.0], [U, 0.0, 0.0, 0.0]])
+    return nrm.reshape(-1), tag.reshape(-1), table
+
+
+def channel_walls(shape, u_top=0.0):
+    """Regularised walls on the interior rows y = 0 (at rest) and y = ny - 1
+    (moving at (u_top, 0, 0)): plane Couette or, with a force, Poiseuille."""
+    nx, ny, nz = shape
+    nrm = np.zeros((nz, ny, nx), dtype=np.uint8)
+    tag = np.zeros((nz, ny, nx), dtype=np.uint16)
+    nrm[:, 0, :] = NRM_YM
+    nrm[:, ny - 1, :] = NRM_YP
+    tag[:, ny - 1, :] = 1
+    table = np.array([[0.0, 0.0, 0.0, 0.0], [u_top, 0.0, 0.0, 0.0]])
+    return nrm.reshape(-1), tag.reshape(-1), table
+
+
+def box_walls(shape, U):
+    """A closed 3-D box: every face node a regularised wall (the face y = ny-1
+    moving at (U, 0, 0), the others at rest), every node on an edge or vertex
+    a corner at rest.  Returns interior (normals, tags) x fastest and the
+    velocity table."""
+    nx, ny, nz = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    faces = [(x == nx - 1, NRM_XP), (x == 0, NRM_XM), (y == ny - 1, NRM_YP), (y == 0, NRM_YM),
+             (z == nz - 1, NRM_ZP), (z == 0, NRM_ZM)]
+    nrm = np.zeros(x.shape, dtype=np.uint8)
+    count = np.zeros(x.shape, dtype=np.int32)
+    for m, code in faces:
+        nrm[m] = code
+        count += m
+    tag = np.zeros(x.shape, dtype=np.uint16)
+    tag[(y == ny - 1) & (count == 1)] = 1
+    nrm[count > 1] = NRM_CORNER
+    table = np.array([[0.0, 0.0, 0.0, 0.0], [U, 0.0, 0.0, 0.0]])
+    return nrm.reshape(-1), tag.reshape(-1), table

@@ -22,6 +22,12 @@ struct Dom {
   int np;          // n_padded
 };
 
+// Fluid, or a regularised-wall node, which collides like one
+// (FluidSolver.hpp:557, 1054, 1215).
+__host__ __device__ __forceinline__ bool fluid_like(uint8_t fl) {
+  return fl == M3LB_FLUID || fl == M3LB_REGWALL;
+}
+
 struct StepArgs {
   void* f;
   uint64_t stride;
@@ -444,7 +450,7 @@ __device__ __forceinline__ void taylor_green_at(const Dom& d, int n, double u0, 
 template <class K>
 __device__ __forceinline__ void seed_value_of(const SeedArgs& a, int n, typename K::R* v) {
   using R = typename K::R;
-  const bool fluid = a.flags[n] == M3LB_FLUID;
+  const bool fluid = fluid_like(a.flags[n]);
   if (a.mode == kSeedUniform) {
     if (fluid) seed<K>(v, R(a.rho0), R(a.u0[0]), R(a.u0[1]), R(a.u0[2]));
     else seed<K>(v, R(a.rho0), R(0), R(0), R(0));
@@ -538,6 +544,12 @@ struct ReadArgs {
   float inv_scale[27];
   double omega, omega_m, omega_b, F[3];
   const void* ff;  // per-node force planes (FIELD only)
+  // regularised walls (null without): normal code, table index, corner rho
+  // (compute type) per padded node, and the (ux, uy, uz, rho) table
+  const uint8_t* bc_nrm;
+  const uint16_t* bc_tag;
+  const void* bc_rho;
+  const double* wtab;
 };
 
 __device__ __forceinline__ int shift_axis(int v, int h, int n, int s, int per) {
@@ -615,6 +627,10 @@ __device__ __forceinline__ void interior_coords(const Dom& d, int n, int& x, int
 }
 
 template <class K>
+__device__ void wall_macro(const ReadArgs& a, unsigned n, double& r, double& ux, double& uy,
+                           double& uz);
+
+template <class K>
 __global__ void __launch_bounds__(kBlock) k_macro(ReadArgs a) {
   using R = typename K::R;
   using T = typename K::T;
@@ -636,6 +652,8 @@ __global__ void __launch_bounds__(kBlock) k_macro(ReadArgs a) {
     else load_node<K, 1>(A, a.stride, unsigned(n), o, a.inv_scale, f);
     const Macro<R> m = macroscopic<K>(f, p);
     r = double(density<K>(m)); ux = double(m.ux); uy = double(m.uy); uz = double(m.uz);
+  } else if (a.flags[n] == M3LB_REGWALL && a.bc_nrm) {
+    wall_macro<K>(a, unsigned(n), r, ux, uy, uz);
   }
   if (a.out2[0]) a.out2[0][n] = r;
   if (a.out2[1]) a.out2[1][n] = ux;
@@ -682,7 +700,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mass(ReadArgs a) {
   const Dom& d = a.d;
   double acc[4] = {0, 0, 0, 0};
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < d.np; n += gridDim.x * kBlock) {
-    if (a.flags[n] != M3LB_FLUID) continue;
+    if (!fluid_like(a.flags[n])) continue;
     int x, y, z;
     interior_coords<K, 0>(d, n, x, y, z);
     const Offs o = offsets(d, x, y, z);
@@ -712,6 +730,310 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mass(ReadArgs a) {
 // post-collision f_i that halfway bounce-back returned -- unshifted in the
 // compute type as unshift<C> does.  body: padded bytes, nonzero = in the body.
 // Three FP64 partials per block.
+// ---------------------------------------------------------------------------
+// Regularised velocity walls (Latt et al. 2008): src/boundary/Regularized.hpp
+// with FluidSolver.hpp:663-710 (corner density), 793-1033 (the wall path of
+// run_step) and 1076-1185 (macro_kernel).  Straight walls and corners.  Every
+// node reads and writes only its own slots, so the wall nodes are updated by
+// k_walls over a compact list in the same step as k_step updates the fluid
+// nodes (which skips them): the result is the reference's single-pass step.
+// k_walls runs first, so a corner's finite-difference stencil reads a fluid
+// neighbour's incoming set from before the step.
+// ---------------------------------------------------------------------------
+constexpr int kNrmCorner = 7;
+
+struct WallArgs {
+  void* f;
+  uint64_t stride;
+  const uint8_t* flags;
+  Dom d;
+  const unsigned* list;  // padded index of every wall node, ascending
+  int nwall;
+  int fd_corners;
+  const uint8_t* bc_nrm;
+  const uint16_t* bc_tag;
+  const uint32_t* bc_unk;  // bit i: direction i did not stream from a fluid-like node
+  const uint8_t* bc_ext;   // corners: extrapolation axis 1..6, 0 = none
+  void* bc_rho;            // compute type
+  const double* wtab;      // (ux, uy, uz, rho) per table entry
+  double omega, omega_m, omega_b, F[3];
+  const void* ff;
+  float scale[27], inv_scale[27];
+};
+
+__host__ __device__ __forceinline__ void normal_of(int code, int n[3]) {
+  n[0] = n[1] = n[2] = 0;
+  if (code == 1) n[0] = 1;
+  else if (code == 2) n[0] = -1;
+  else if (code == 3) n[1] = 1;
+  else if (code == 4) n[1] = -1;
+  else if (code == 5) n[2] = 1;
+  else if (code == 6) n[2] = -1;
+}
+
+// EquilibriumOf<Collision>::eq (Regularized.hpp:14-19): the operator's own
+// equilibrium for BGK and TRT, the product form for the moment operators.
+template <class K, int i>
+__device__ __forceinline__ typename K::R wall_eq(typename K::R rho, typename K::R ux,
+                                                 typename K::R uy, typename K::R uz) {
+  using R = typename K::R;
+  constexpr bool product = K::OP != M3LB_OP_BGK && K::OP != M3LB_OP_TRT;
+  R phi;
+  if constexpr (product) phi = phi_product<R, K::Q, i>(ux, uy, uz);
+  else phi = phi_second<R, K::Q, i>(ux, uy, uz);
+  return weight<K>(i) * rho * (R(1) + phi);
+}
+
+// Regularized::density
+template <class K>
+__device__ __forceinline__ typename K::R bc_density(const typename K::R* f, const int nrm[3],
+                                                    const typename K::R u[3]) {
+  using R = typename K::R;
+  R out = R(0), tang = R(0);
+  sfor<K::Q>([&](auto I) {
+    constexpr int i = I;
+    const int cn = cvel<K::Q>(i, 0) * nrm[0] + cvel<K::Q>(i, 1) * nrm[1] + cvel<K::Q>(i, 2) * nrm[2];
+    if (cn > 0) out += f[i];
+    else if (cn == 0) tang += f[i];
+  });
+  const R uperp = u[0] * R(nrm[0]) + u[1] * R(nrm[1]) + u[2] * R(nrm[2]);
+  return (R(2) * out + tang) / (R(1) + uperp);
+}
+
+template <class K, int i>
+__device__ __forceinline__ typename K::R qpi(const typename K::R Pi[6]) {
+  using R = typename K::R;
+  const R cs2v = R(1) / R(3);
+  const R cx = R(cvel<K::Q>(i, 0)), cy = R(cvel<K::Q>(i, 1)), cz = R(cvel<K::Q>(i, 2));
+  return (cx * cx - cs2v) * Pi[0] + (cy * cy - cs2v) * Pi[1] +
+         ((K::D == 3) ? (cz * cz - cs2v) * Pi[2] : R(0)) +
+         R(2) * (cx * cy * Pi[3] + ((K::D == 3) ? (cx * cz * Pi[4] + cy * cz * Pi[5]) : R(0)));
+}
+
+// Regularized::apply: unknown directions from the bounced non-equilibrium
+// part (with the Guo odd part), then every direction rebuilt from (rho, u, Pi).
+template <class K>
+__device__ __forceinline__ void bc_apply(typename K::R* f, typename K::R rho,
+                                         const typename K::R u[3], uint32_t unk,
+                                         const typename K::R Fv[3]) {
+  using R = typename K::R;
+  constexpr int Q = K::Q;
+  const R cs2v = R(1) / R(3), cs4v = cs2v * cs2v;
+  R feq[Q];
+  sfor<Q>([&](auto I) { feq[int(I)] = wall_eq<K, int(I)>(rho, u[0], u[1], u[2]); });
+  const R hodd = R(0.5) / cs2v;
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    constexpr int io = opp<Q>(i);
+    if (!(unk & (1u << i))) return;
+    const R cF = R(cvel<Q>(i, 0)) * Fv[0] + R(cvel<Q>(i, 1)) * Fv[1] + R(cvel<Q>(i, 2)) * Fv[2];
+    const R odd = weight<K>(i) * hodd * cF;
+    f[i] = (unk & (1u << io)) ? feq[i] - odd : feq[i] + (f[io] - feq[io]) - R(2) * odd;
+  });
+  R Pi[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    const R dlt = f[i] - feq[i];
+    const R cx = R(cvel<Q>(i, 0)), cy = R(cvel<Q>(i, 1)), cz = R(cvel<Q>(i, 2));
+    Pi[0] += dlt * cx * cx; Pi[1] += dlt * cy * cy; Pi[2] += dlt * cz * cz;
+    Pi[3] += dlt * cx * cy; Pi[4] += dlt * cx * cz; Pi[5] += dlt * cy * cz;
+  });
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    const R cF = R(cvel<Q>(i, 0)) * Fv[0] + R(cvel<Q>(i, 1)) * Fv[1] + R(cvel<Q>(i, 2)) * Fv[2];
+    f[i] = feq[i] - weight<K>(i) * hodd * cF + weight<K>(i) / (R(2) * cs4v) * qpi<K, i>(Pi);
+  });
+}
+
+// Regularized::apply_with_stress
+template <class K>
+__device__ __forceinline__ void bc_apply_with_stress(typename K::R* f, typename K::R rho,
+                                                     const typename K::R u[3],
+                                                     const typename K::R Pi[6]) {
+  using R = typename K::R;
+  const R cs2v = R(1) / R(3), cs4v = cs2v * cs2v;
+  sfor<K::Q>([&](auto I) {
+    constexpr int i = I;
+    f[i] = wall_eq<K, i>(rho, u[0], u[1], u[2]) + weight<K>(i) / (R(2) * cs4v) * qpi<K, i>(Pi);
+  });
+}
+
+template <class K>
+__device__ __forceinline__ void wall_velocity(const WallArgs& a, unsigned n, typename K::R u[3]) {
+  const double* w = a.wtab + 4 * size_t(a.bc_tag[n]);
+  u[0] = typename K::R(w[0]);
+  u[1] = typename K::R(w[1]);
+  u[2] = typename K::R(w[2]);
+}
+
+// The incoming set of padded node n at parity P, unshifted to f.
+template <class K, int P>
+__device__ __forceinline__ void load_raw(const void* f, uint64_t stride, const Dom& d,
+                                         unsigned n, const float* inv, typename K::R* out) {
+  int x, y, z;
+  interior_coords<K, 0>(d, int(n), x, y, z);
+  const Offs o = offsets(d, x, y, z);
+  load_node<K, P>(static_cast<const typename K::T*>(f), stride, n, o, inv, out);
+  if constexpr (K::SHIFTED) sfor<K::Q>([&](auto I) { out[int(I)] += weight<K>(int(I)); });
+}
+
+// corner_density<P>, first half: rho of every straight-wall node.
+template <class K, int P>
+__global__ void __launch_bounds__(kBlock) k_wall_density(WallArgs a) {
+  using R = typename K::R;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.nwall) return;
+  const unsigned n = a.list[k];
+  const int code = a.bc_nrm[n];
+  if (code == kNrmCorner || code == 0) return;
+  R f[K::Q];
+  load_raw<K, P>(a.f, a.stride, a.d, n, a.inv_scale, f);
+  int nrm[3];
+  normal_of(code, nrm);
+  R uw[3];
+  wall_velocity<K>(a, n, uw);
+  static_cast<R*>(a.bc_rho)[n] = bc_density<K>(f, nrm, uw);
+}
+
+// corner_density<P>, second half: each corner 2 rho(n1) - rho(n2) along its
+// axis, or 1 without a stencil.
+template <class K>
+__global__ void __launch_bounds__(kBlock) k_corner_density(WallArgs a) {
+  using R = typename K::R;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.nwall) return;
+  const unsigned n = a.list[k];
+  if (a.bc_nrm[n] != kNrmCorner) return;
+  R* rho = static_cast<R*>(a.bc_rho);
+  const int e = a.bc_ext[n];
+  if (e == 0) {
+    rho[n] = R(1);
+    return;
+  }
+  const Dom& d = a.d;
+  int x, y, z;
+  interior_coords<K, 0>(d, int(n), x, y, z);
+  const int ax = (e - 1) >> 1, sg = (e & 1) ? 1 : -1;  // 1:+x 2:-x 3:+y 4:-y 5:+z 6:-z
+  int v[3] = {0, 0, 0};
+  v[ax] = sg;
+  auto id = [&](int xx, int yy, int zz) {
+    return unsigned(((zz + d.hz) * d.sy + (yy + d.hy)) * d.sx + xx + d.hx);
+  };
+  const unsigned n1 = id(x + v[0], y + v[1], z + v[2]);
+  const unsigned n2 = id(x + 2 * v[0], y + 2 * v[1], z + 2 * v[2]);
+  rho[n] = R(2) * rho[n1] - rho[n2];
+}
+
+// The wall path of run_step for one wall node: rebuild, collide, store.
+template <class K, int P>
+__global__ void __launch_bounds__(kBlock) k_walls(WallArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  constexpr int Q = K::Q;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.nwall) return;
+  const unsigned n = a.list[k];
+  const Dom& d = a.d;
+  int x, y, z;
+  interior_coords<K, 0>(d, int(n), x, y, z);
+  const Offs o = offsets(d, x, y, z);
+  T* A = static_cast<T*>(a.f);
+  R f[Q];
+  load_node<K, P>(A, a.stride, n, o, a.inv_scale, f);
+  if constexpr (K::SHIFTED) sfor<Q>([&](auto I) { f[int(I)] += weight<K>(int(I)); });
+  const int code = a.bc_nrm[n];
+  int nrm[3];
+  normal_of(code, nrm);
+  R uw[3];
+  wall_velocity<K>(a, n, uw);
+  Params<R> p;
+  p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
+  p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
+  force_at<K>(p, a.ff, a.stride, n);
+  R Fv[3] = {R(0), R(0), R(0)};
+  if constexpr (K::FORCE) { Fv[0] = p.F[0]; Fv[1] = p.F[1]; Fv[2] = p.F[2]; }
+  const R rw = (code == kNrmCorner) ? static_cast<const R*>(a.bc_rho)[n] : bc_density<K>(f, nrm, uw);
+  if (a.fd_corners && code == kNrmCorner) {
+    const int ic[3] = {x, y, z}, nn[3] = {d.nx, d.ny, d.nz}, per[3] = {d.px, d.py, d.pz};
+    R grad[3][3] = {{R(0), R(0), R(0)}, {R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
+    auto vel_at = [&](int cx, int cy, int cz, R out[3]) {
+      const unsigned m = unsigned(((cz + d.hz) * d.sy + (cy + d.hy)) * d.sx + cx + d.hx);
+      if (a.flags[m] == M3LB_REGWALL) {
+        wall_velocity<K>(a, m, out);
+        return;
+      }
+      R g[Q];
+      load_raw<K, P>(a.f, a.stride, d, m, a.inv_scale, g);
+      R sm = R(0), mx = R(0), my = R(0), mz = R(0);
+      sfor<Q>([&](auto I) {
+        constexpr int i = I;
+        sm += g[i];
+        mx += g[i] * R(cvel<Q>(i, 0));
+        my += g[i] * R(cvel<Q>(i, 1));
+        mz += g[i] * R(cvel<Q>(i, 2));
+      });
+      const R ir = R(1) / sm;
+      out[0] = mx * ir; out[1] = my * ir; out[2] = mz * ir;
+    };
+    for (int ax = 0; ax < K::D; ++ax) {
+      if (nn[ax] < 3) continue;
+      int dir = 0;
+      if (!per[ax]) {
+        if (ic[ax] == 0) dir = 1;
+        else if (ic[ax] == nn[ax] - 1) dir = -1;
+      }
+      int c1[3] = {ic[0], ic[1], ic[2]}, c2[3] = {ic[0], ic[1], ic[2]};
+      R u1[3], u2[3];
+      if (dir != 0) {
+        c1[ax] += dir; c2[ax] += 2 * dir;
+        vel_at(c1[0], c1[1], c1[2], u1);
+        vel_at(c2[0], c2[1], c2[2], u2);
+        for (int b = 0; b < K::D; ++b)
+          grad[ax][b] = R(dir) * (R(-1.5) * uw[b] + R(2) * u1[b] - R(0.5) * u2[b]);
+      } else {
+        c1[ax] = (ic[ax] + 1) % nn[ax];
+        c2[ax] = (ic[ax] - 1 + nn[ax]) % nn[ax];
+        vel_at(c1[0], c1[1], c1[2], u1);
+        vel_at(c2[0], c2[1], c2[2], u2);
+        for (int b = 0; b < K::D; ++b) grad[ax][b] = R(0.5) * (u1[b] - u2[b]);
+      }
+    }
+    // Regularized::stress_from_gradient with the shear rate
+    const R kf = -R(2) * (R(1) / R(3)) * rw / p.omega;
+    const R Pi[6] = {kf * grad[0][0], kf * grad[1][1], kf * grad[2][2],
+                     kf * R(0.5) * (grad[0][1] + grad[1][0]), kf * R(0.5) * (grad[0][2] + grad[2][0]),
+                     kf * R(0.5) * (grad[1][2] + grad[2][1])};
+    bc_apply_with_stress<K>(f, rw, uw, Pi);
+  } else {
+    bc_apply<K>(f, rw, uw, a.bc_unk[n], Fv);
+  }
+  if constexpr (K::SHIFTED) sfor<Q>([&](auto I) { f[int(I)] -= weight<K>(int(I)); });
+  collide_fused<K>(f, p);
+  store_node<K, P>(A, a.stride, n, o, a.scale, f);
+}
+
+// macro_kernel's RegWall path (FluidSolver.hpp:1076-1185): the imposed
+// velocity and the closure density; a corner reports the last pass's rho.
+template <class K>
+__device__ void wall_macro(const ReadArgs& a, unsigned n, double& r, double& ux, double& uy,
+                           double& uz) {
+  using R = typename K::R;
+  const int code = a.bc_nrm[n];
+  const double* w = a.wtab + 4 * size_t(a.bc_tag[n]);
+  const R uw[3] = {R(w[0]), R(w[1]), R(w[2])};
+  ux = double(uw[0]); uy = double(uw[1]); uz = double(uw[2]);
+  if (code == kNrmCorner) {
+    r = double(static_cast<const R*>(a.bc_rho)[n]);
+    return;
+  }
+  R f[K::Q];
+  if (a.parity == 0) load_raw<K, 0>(a.f, a.stride, a.d, n, a.inv_scale, f);
+  else load_raw<K, 1>(a.f, a.stride, a.d, n, a.inv_scale, f);
+  int nrm[3];
+  normal_of(code, nrm);
+  r = double(bc_density<K>(f, nrm, uw));
+}
+
 template <class K>
 __global__ void __launch_bounds__(kBlock) k_momentum_exchange(ReadArgs a, const uint8_t* body) {
   using R = typename K::R;

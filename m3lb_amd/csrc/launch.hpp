@@ -21,6 +21,9 @@ struct KernelSet {
   void (*absmax)(const ReadArgs&, unsigned grid, hipStream_t);
   void (*rescale)(void* f, uint64_t stride, int np, const Ratio&, unsigned grid, hipStream_t);
   void (*exchange)(const ReadArgs&, const uint8_t* body, unsigned grid, hipStream_t);
+  // regularised walls: corner-density pass (both halves) and the wall update
+  void (*wall_density)(const WallArgs&, int parity, hipStream_t);
+  void (*walls)(const WallArgs&, int parity, hipStream_t);
   // batched server step; null for the identity op and 16-bit storage
   void (*batched)(const BatchArgs&, unsigned grid, hipStream_t);
   int elem_bytes;    // bytes per stored population
@@ -66,6 +69,21 @@ struct Launch {
                       hipStream_t s) {
     if constexpr (K::C::scaled) k_rescale<K><<<grid, kBlock, 0, s>>>(f, stride, np, r);
   }
+  static void wall_density(const WallArgs& a, int parity, hipStream_t s) {
+    if constexpr (K::OP != M3LB_OP_NULL) {
+      const unsigned g = blocks_for(a.nwall);
+      if (parity == 0) k_wall_density<K, 0><<<g, kBlock, 0, s>>>(a);
+      else k_wall_density<K, 1><<<g, kBlock, 0, s>>>(a);
+      k_corner_density<K><<<g, kBlock, 0, s>>>(a);
+    }
+  }
+  static void walls(const WallArgs& a, int parity, hipStream_t s) {
+    if constexpr (K::OP != M3LB_OP_NULL) {
+      const unsigned g = blocks_for(a.nwall);
+      if (parity == 0) k_walls<K, 0><<<g, kBlock, 0, s>>>(a);
+      else k_walls<K, 1><<<g, kBlock, 0, s>>>(a);
+    }
+  }
   static void exchange(const ReadArgs& a, const uint8_t* body, unsigned grid, hipStream_t s) {
     k_momentum_exchange<K><<<grid, kBlock, 0, s>>>(a, body);
   }
@@ -78,7 +96,7 @@ struct Launch {
   static const KernelSet* table() {
     static const KernelSet t = {step,     seed,      gather, storage,
                                 macro,    reduce_mass, sum_exact, absmax,
-                                rescale,  exchange,
+                                rescale,  exchange, wall_density, walls,
                                 kBatched ? batched : nullptr,
                                 int(sizeof(typename K::T)), int(sizeof(typename K::R))};
     return &t;

@@ -101,6 +101,17 @@ struct m3lb_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int remap = 64;  // step-kernel tile order (kernels.hpp tile_of)
   void* ff_d = nullptr;  // M3LB_FORCE_FIELD: fx, fy, fz planes of `stride` (compute type)
+  // regularised walls (m3lb_set_regularized_walls)
+  int nwall = 0;
+  bool has_corners = false;
+  int fd_corners = 1;  // fd_corners_ = has_shear_omega<Collision> (FluidSolver.hpp:1287)
+  uint8_t* nrm_d = nullptr;
+  uint16_t* tag_d = nullptr;
+  uint32_t* unk_d = nullptr;
+  uint8_t* ext_d = nullptr;
+  void* wrho_d = nullptr;
+  double* wtab_d = nullptr;
+  unsigned* wlist_d = nullptr;
 };
 
 namespace {
@@ -153,7 +164,48 @@ m3lb::ReadArgs read_args(const m3lb_solver* s) {
   for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
   a.ff = s->ff_d;
   for (int q = 0; q < 27; ++q) a.inv_scale[q] = s->inv_scale[q];
+  if (s->nwall) {
+    a.bc_nrm = s->nrm_d;
+    a.bc_tag = s->tag_d;
+    a.bc_rho = s->wrho_d;
+    a.wtab = s->wtab_d;
+  }
   return a;
+}
+
+m3lb::WallArgs wall_args(const m3lb_solver* s) {
+  m3lb::WallArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.f = s->f;
+  a.stride = s->stride;
+  a.flags = s->flags_d;
+  a.d = s->d;
+  a.list = s->wlist_d;
+  a.nwall = s->nwall;
+  a.fd_corners = s->fd_corners;
+  a.bc_nrm = s->nrm_d;
+  a.bc_tag = s->tag_d;
+  a.bc_unk = s->unk_d;
+  a.bc_ext = s->ext_d;
+  a.bc_rho = s->wrho_d;
+  a.wtab = s->wtab_d;
+  a.omega = s->cfg.omega;
+  a.omega_m = s->cfg.omega_minus;
+  a.omega_b = s->cfg.omega_bulk;
+  for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
+  a.ff = s->ff_d;
+  for (int q = 0; q < 27; ++q) { a.scale[q] = s->scale[q]; a.inv_scale[q] = s->inv_scale[q]; }
+  return a;
+}
+
+void free_walls(m3lb_solver* s) {
+  void* ps[] = {s->nrm_d, s->tag_d, s->unk_d, s->ext_d, s->wrho_d, s->wtab_d, s->wlist_d};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  s->nrm_d = nullptr; s->tag_d = nullptr; s->unk_d = nullptr; s->ext_d = nullptr;
+  s->wrho_d = nullptr; s->wtab_d = nullptr; s->wlist_d = nullptr;
+  s->nwall = 0;
+  s->has_corners = false;
 }
 
 m3lb::SeedArgs seed_args(const m3lb_solver* s) {
@@ -328,6 +380,7 @@ int m3lb_destroy(m3lb_solver* s) {
   if (s->f) hipFree(s->f);
   if (s->flags_d) hipFree(s->flags_d);
   if (s->ff_d) hipFree(s->ff_d);
+  free_walls(s);
   for (void* p : s->macro_d)
     if (p) hipFree(p);
   if (s->partial_d) hipFree(s->partial_d);
@@ -340,6 +393,11 @@ int m3lb_destroy(m3lb_solver* s) {
 
 int m3lb_set_geometry(m3lb_solver* s, const uint8_t* fl) {
   if (!s || !fl) return fail(M3LB_EINVAL, "null argument");
+  // every interior flag is rewritten, so regularised walls set earlier are gone
+  if (s->nwall) {
+    HIP_OK(hipStreamSynchronize(s->stream));
+    free_walls(s);
+  }
   const m3lb::Dom& d = s->d;
   bool geom = false;
   size_t k = 0;
@@ -432,10 +490,18 @@ int m3lb_step(m3lb_solver* s, int32_t nsteps, int32_t store_macroscopic) {
   const unsigned rows_per_block = unsigned(m3lb::kBlock >> a.bx_log2);
   const unsigned grid = a.chunks > 1 ? unsigned(a.rows) * unsigned(a.chunks)
                                      : unsigned((a.rows + rows_per_block - 1) / rows_per_block);
+  const m3lb::WallArgs wa = s->nwall ? wall_args(s) : m3lb::WallArgs{};
   for (int k = 0; k < nsteps; ++k) {
     const bool last = k == nsteps - 1;
     for (int j = 0; j < 4; ++j) a.macro[j] = (store_macroscopic && last) ? s->macro_d[j] : nullptr;
-    s->ks->step(a, int(s->t % 2), s->geometry, grid, s->stream);
+    const int parity = int(s->t % 2);
+    if (s->nwall) {
+      // FluidSolver::step (:583): the corner-density pass, then the step; the
+      // wall nodes' share of the step runs first (see kernels.hpp, k_walls)
+      if (s->has_corners) s->ks->wall_density(wa, parity, s->stream);
+      s->ks->walls(wa, parity, s->stream);
+    }
+    s->ks->step(a, parity, s->geometry, grid, s->stream);
     ++s->t;
   }
   s->macro_valid = store_macroscopic != 0;
@@ -750,6 +816,118 @@ int m3lb_storage(m3lb_solver* s, double* out) { return read_pops(s, out, false);
 int m3lb_timestep(const m3lb_solver* s, uint64_t* t) {
   if (!s || !t) return fail(M3LB_EINVAL, "null argument");
   *t = s->t;
+  return M3LB_OK;
+}
+
+int m3lb_set_regularized_walls(m3lb_solver* s, const uint8_t* normals, const uint16_t* tags,
+                                const m3lb_wall* table, int32_t ntab) {
+  if (!s || !normals || !tags || !table) return fail(M3LB_EINVAL, "null argument");
+  if (ntab < 1 || ntab > 65536) return fail(M3LB_EINVAL, "the wall table holds 1..65536 entries");
+  if (s->cfg.op == M3LB_OP_NULL) return fail(M3LB_EINVAL, "the identity collision has no walls");
+  const m3lb::Dom& d = s->d;
+  const size_t np = size_t(d.np);
+  auto id = [&](int x, int y, int z) { return size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx); };
+  std::vector<uint8_t> nrm(np, 0), ext(np, 0);
+  std::vector<uint16_t> tag(np, 0);
+  std::vector<uint32_t> unk(np, 0);
+  size_t k = 0;
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x, ++k) {
+        if (normals[k] > m3lb::kNrmCorner) return fail(M3LB_EINVAL, "normal codes are 0..7");
+        if (normals[k] == 0) continue;
+        if (tags[k] >= ntab) return fail(M3LB_EINVAL, "wall tag outside the table");
+        nrm[id(x, y, z)] = normals[k];
+        tag[id(x, y, z)] = tags[k];
+      }
+  // corners: an axis direction whose next two nodes are straight-wall nodes
+  static const int dirs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  bool corners = false;
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x) {
+        if (nrm[id(x, y, z)] != m3lb::kNrmCorner) continue;
+        corners = true;
+        for (int q = 0; q < (d.nz > 1 ? 6 : 4); ++q) {
+          const int x1 = x + dirs[q][0], y1 = y + dirs[q][1], z1 = z + dirs[q][2];
+          const int x2 = x + 2 * dirs[q][0], y2 = y + 2 * dirs[q][1], z2 = z + 2 * dirs[q][2];
+          if (x1 < 0 || y1 < 0 || z1 < 0 || x2 < 0 || y2 < 0 || z2 < 0) continue;
+          if (x1 >= d.nx || y1 >= d.ny || z1 >= d.nz || x2 >= d.nx || y2 >= d.ny || z2 >= d.nz) continue;
+          const int a = nrm[id(x1, y1, z1)], b = nrm[id(x2, y2, z2)];
+          if (a != 0 && a != m3lb::kNrmCorner && b != 0 && b != m3lb::kNrmCorner) {
+            ext[id(x, y, z)] = uint8_t(q + 1);
+            break;
+          }
+        }
+      }
+  // flags: the wall nodes become RegWall
+  for (size_t n = 0; n < np; ++n)
+    if (nrm[n]) s->flags_h[n] = M3LB_REGWALL;
+  // unknown directions: source p - c_i outside a non-periodic extent, or an
+  // Excluded or Solid node (FluidSolver.hpp:420-455)
+  const int next[3] = {d.nx, d.ny, d.nz}, per[3] = {d.px, d.py, d.pz};
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x) {
+        const size_t n = id(x, y, z);
+        if (!nrm[n]) continue;
+        uint32_t m = 0;
+        for (int i = 0; i < s->Q; ++i) {
+          const int c[3] = {s->Q == 9 ? m3lb::Lat<9>::cx[i] : m3lb::Lat<27>::cx[i],
+                            s->Q == 9 ? m3lb::Lat<9>::cy[i] : m3lb::Lat<27>::cy[i],
+                            s->Q == 9 ? m3lb::Lat<9>::cz[i] : m3lb::Lat<27>::cz[i]};
+          int src[3] = {x - c[0], y - c[1], z - c[2]};
+          bool outside = false;
+          for (int a = 0; a < 3; ++a) {
+            if (per[a]) src[a] = (src[a] + next[a]) % next[a];
+            else if (src[a] < 0 || src[a] >= next[a]) outside = true;
+          }
+          if (!outside) {
+            const uint8_t sf = s->flags_h[id(src[0], src[1], src[2])];
+            if (sf == M3LB_EXCLUDED || sf == M3LB_SOLID) outside = true;
+          }
+          if (outside) m |= 1u << i;
+        }
+        unk[n] = m;
+      }
+  std::vector<unsigned> list;
+  for (size_t n = 0; n < np; ++n)
+    if (nrm[n]) list.push_back(unsigned(n));
+  std::vector<double> tab(size_t(ntab) * 4);
+  for (int t = 0; t < ntab; ++t) {
+    tab[size_t(t) * 4 + 0] = table[t].ux;
+    tab[size_t(t) * 4 + 1] = table[t].uy;
+    tab[size_t(t) * 4 + 2] = table[t].uz;
+    tab[size_t(t) * 4 + 3] = table[t].rho;
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  free_walls(s);
+  HIP_OK(hipMalloc(&s->nrm_d, np));
+  HIP_OK(hipMalloc(&s->tag_d, np * sizeof(uint16_t)));
+  HIP_OK(hipMalloc(&s->unk_d, np * sizeof(uint32_t)));
+  HIP_OK(hipMalloc(&s->ext_d, np));
+  HIP_OK(hipMalloc(&s->wrho_d, np * size_t(s->ks->real_bytes)));
+  HIP_OK(hipMalloc(&s->wtab_d, tab.size() * sizeof(double)));
+  HIP_OK(hipMalloc(&s->wlist_d, std::max<size_t>(1, list.size()) * sizeof(unsigned)));
+  HIP_OK(hipMemcpy(s->nrm_d, nrm.data(), np, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->tag_d, tag.data(), np * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->unk_d, unk.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->ext_d, ext.data(), np, hipMemcpyHostToDevice));
+  HIP_OK(hipMemset(s->wrho_d, 0, np * size_t(s->ks->real_bytes)));
+  HIP_OK(hipMemcpy(s->wtab_d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (!list.empty())
+    HIP_OK(hipMemcpy(s->wlist_d, list.data(), list.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(s->flags_d, s->flags_h.data(), np, hipMemcpyHostToDevice));
+  s->nwall = int(list.size());
+  s->has_corners = corners;
+  s->geometry = s->geometry || s->nwall > 0;
+  s->macro_valid = false;
+  return M3LB_OK;
+}
+
+int m3lb_set_fd_corners(m3lb_solver* s, int32_t on) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  s->fd_corners = on ? 1 : 0;
   return M3LB_OK;
 }
 

@@ -43,6 +43,10 @@ SIGNATURES = {
     "m3lb_timestep": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     "m3lb_set_force": (ctypes.c_int, [_vp] + [ctypes.c_double] * 3),
     "m3lb_set_force_field": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "m3lb_set_regularized_walls": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8),
+                                                  ctypes.POINTER(ctypes.c_uint16), _dp,
+                                                  ctypes.c_int32]),
+    "m3lb_set_fd_corners": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "m3lb_set_omega": (ctypes.c_int, [_vp] + [ctypes.c_double] * 3),
     "m3lb_get_scales": (ctypes.c_int, [_vp, _dp]),
     "m3lb_set_scales": (ctypes.c_int, [_vp, _dp]),
@@ -252,6 +256,21 @@ class FluidSolver:
         t = ctypes.c_uint64()
         _check(self.L.m3lb_timestep(self.h, ctypes.byref(t)))
         return t.value
+
+    def set_regularized_walls(self, normals, tags, table):
+        """m3lb_set_regularized_walls: interior normal codes and table indices
+        (x fastest) and the (ux, uy, uz, rho) table."""
+        nrm = np.ascontiguousarray(normals, dtype=np.uint8).reshape(-1)
+        tg = np.ascontiguousarray(tags, dtype=np.uint16).reshape(-1)
+        tab = np.ascontiguousarray(table, dtype=np.float64).reshape(-1, 4)
+        n = self.cfg.nx * self.cfg.ny * self.cfg.nz
+        assert nrm.size == n and tg.size == n
+        _check(self.L.m3lb_set_regularized_walls(
+            self.h, nrm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            tg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), _d(tab), int(tab.shape[0])))
+
+    def set_fd_corners(self, on):
+        _check(self.L.m3lb_set_fd_corners(self.h, 1 if on else 0))
 
     def set_force_field(self, fx, fy, fz):
         """m3lb_set_force_field: per-node Guo force, padded arrays."""

@@ -52,6 +52,10 @@ def load():
         "orc_timestep": (ctypes.c_uint64, [vp]),
         "orc_set_force": (None, [vp] + [ctypes.c_double] * 3),
         "orc_set_force_field": (None, [vp, _dp, _dp, _dp]),
+        "orc_set_regularized_walls": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint8),
+                                                     ctypes.POINTER(ctypes.c_uint16), _dp,
+                                                     ctypes.c_int]),
+        "orc_set_fd_corners": (None, [vp, ctypes.c_int]),
         "orc_time_steps": (ctypes.c_double, [vp, ctypes.c_int]),
         "orc_collide_node": (None, [_cfgp, _dp, _dp, _dp]),
         "orc_macroscopic_node": (None, [_cfgp, _dp, _dp]),
@@ -163,6 +167,20 @@ class OracleSolver:
 
     def set_force(self, fx, fy, fz):
         self.L.orc_set_force(self.h, fx, fy, fz)
+
+    def set_regularized_walls(self, normals, tags, table):
+        nrm = np.ascontiguousarray(normals, dtype=np.uint8).reshape(-1)
+        tg = np.ascontiguousarray(tags, dtype=np.uint16).reshape(-1)
+        tab = np.ascontiguousarray(table, dtype=np.float64).reshape(-1, 4)
+        rc = self.L.orc_set_regularized_walls(
+            self.h, nrm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            tg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), tab.ctypes.data_as(_dp),
+            int(tab.shape[0]))
+        if rc != 0:
+            raise ValueError("invalid regularised wall specification")
+
+    def set_fd_corners(self, on):
+        self.L.orc_set_fd_corners(self.h, 1 if on else 0)
 
     def set_force_field(self, fx, fy, fz):
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (fx, fy, fz)]

@@ -64,6 +64,29 @@ void orc_set_force(orc_solver* s, double fx, double fy, double fz) {
 void orc_set_force_field(orc_solver* s, const double* fx, const double* fy, const double* fz) {
   DISPATCH(s, set_force_field, fx, fy, fz);
 }
+int orc_set_regularized_walls(orc_solver* s, const uint8_t* normals, const uint16_t* tags,
+                              const double* table, int ntab) {
+  return DISPATCH(s, set_regularized_walls, normals, tags, table, ntab);
+}
+void orc_set_fd_corners(orc_solver* s, int on) { DISPATCH(s, set_fd_corners, on); }
+void orc_set_force_correction(orc_solver* s, int on) { DISPATCH(s, set_force_correction, on); }
+void orc_set_outflow_order(orc_solver* s, int k) { DISPATCH(s, set_outflow_order, k); }
+void orc_set_outflow_density(orc_solver* s, double r) { DISPATCH(s, set_outflow_density, r); }
+void orc_set_wall_velocity_scale(orc_solver* s, double sc) { DISPATCH(s, set_wall_velocity_scale, sc); }
+void orc_set_wall_density(orc_solver* s, double r) { DISPATCH(s, set_wall_density, r); }
+void orc_set_wall_velocity_field(orc_solver* s, const double* ux, const double* uy,
+                                 const double* uz) {
+  DISPATCH(s, set_wall_velocity_field, ux, uy, uz);
+}
+void orc_set_wall_velocity_field_scale(orc_solver* s, double sc) {
+  DISPATCH(s, set_wall_velocity_field_scale, sc);
+}
+int orc_set_specular_walls(orc_solver* s, const uint8_t* normals) {
+  return DISPATCH(s, set_specular_walls, normals);
+}
+int orc_set_specular_nodes(orc_solver* s, const uint8_t* faces) {
+  return DISPATCH(s, set_specular_nodes, faces);
+}
 double orc_time_steps(orc_solver* s, int nsteps) { return DISPATCH(s, time_steps, nsteps); }
 
 #define NODE(name, cfg, ...) \

@@ -50,6 +50,25 @@ uint64_t orc_timestep(const orc_solver* s);
 void orc_set_force(orc_solver* s, double fx, double fy, double fz);
 /* Per-node Guo force for M3LB_FORCE_FIELD configs: three padded planes. */
 void orc_set_force_field(orc_solver* s, const double* fx, const double* fy, const double* fz);
+/* Regularised walls: normal code per INTERIOR node (0 none, 1..6 =
+ * +x,-x,+y,-y,+z,-z outward normal, 7 corner, 8..10 the outflow codes),
+ * table index per node, and the table of (ux, uy, uz, rho).  Returns 0 or -1. */
+int orc_set_regularized_walls(orc_solver* s, const uint8_t* normals, const uint16_t* tags,
+                              const double* table, int ntab);
+void orc_set_fd_corners(orc_solver* s, int on);
+void orc_set_force_correction(orc_solver* s, int on);
+void orc_set_outflow_order(orc_solver* s, int k);
+void orc_set_outflow_density(orc_solver* s, double r);
+void orc_set_wall_velocity_scale(orc_solver* s, double sc);
+void orc_set_wall_density(orc_solver* s, double r);
+/* one wall velocity per INTERIOR node (x fastest) */
+void orc_set_wall_velocity_field(orc_solver* s, const double* ux, const double* uy,
+                                 const double* uz);
+void orc_set_wall_velocity_field_scale(orc_solver* s, double sc);
+/* Specular ghost cells: axis normal 1..6 per interior node.  Returns 0 or -1. */
+int orc_set_specular_walls(orc_solver* s, const uint8_t* normals);
+/* On-node specular nodes: face mask per interior node.  Returns 0 or -1. */
+int orc_set_specular_nodes(orc_solver* s, const uint8_t* faces);
 /* Wall-clock seconds for nsteps steps after one untimed warm-up step,
  * timed as src/app/main.cpp:41-47 does. */
 double orc_time_steps(orc_solver* s, int nsteps);

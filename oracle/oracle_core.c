@@ -551,6 +551,30 @@ typedef struct {
   uint64_t t;
   int nthreads;
   REAL* ff; /* FORCE_FIELD: fx, fy, fz planes of n_padded values, or NULL */
+  /* regularised walls (FluidSolver.hpp:1259-1290), per padded node */
+  uint8_t* bc_nrm;
+  uint16_t* bc_tag;
+  uint32_t* bc_unk;
+  uint8_t* bc_ext;
+  REAL* bc_rho;
+  double* wtab; /* (ux, uy, uz, rho) per table entry */
+  int ntab;
+  int has_corners, fd_corners;
+  int64_t* walls;
+  int64_t n_walls;
+  /* outflow codes: donor node and its axis face (NrmOutFree), the table as
+   * given (set_wall_velocity_scale), the per-node wall velocity field */
+  int64_t* bc_don;
+  uint8_t* bc_onrm;
+  double* wtab0;
+  REAL* wall_un; /* ux, uy, uz planes of n_padded values, or NULL */
+  REAL un_scale;
+  int force_bc, out_order;
+  REAL outflow_rho;
+  /* specular walls (FluidSolver.hpp:1262-1270): ghost normal, on-node faces */
+  uint8_t* spec_nrm;
+  uint8_t* spec_faces;
+  int mirror[3][27]; /* MirrorTable<L>::m (Specular.hpp:104-121) */
 } S;
 
 /* The collision policy at node n: for a per-node force (a forcing policy
@@ -590,16 +614,65 @@ static inline REAL tl_load(const S* s, const int64_t* j, int64_t n, int i) {
   return F(s, i == 0 ? n : j[opp(i)], i);
 }
 
-/* rebuild_lists, FluidSolver.hpp:157-177 */
+/* rebuild_lists, FluidSolver.hpp:157-177.  The RegWall nodes go to their own
+ * list (they are updated before the rest of the step, see run_step). */
 static void rebuild_lists(S* s) {
-  int64_t k = 0;
+  int64_t k = 0, w = 0;
   for (int64_t n = 0; n < s->d.np; ++n) {
     const uint8_t fl = s->flags[n];
-    if (fl == M3LB_EXCLUDED) continue;
+    if (fl == M3LB_REGWALL) s->walls[w++] = n;
+    if (fl == M3LB_EXCLUDED || fl == M3LB_REGWALL) continue;
     if (fl == M3LB_SOLID && s->streaming == 0) continue;
     s->active[k++] = n;
   }
   s->n_active = k;
+  s->n_walls = w;
+}
+
+/* A node that holds a fluid state (FluidSolver.hpp:557, 575, 1054, 1215). */
+static inline int fluid_like(uint8_t fl) {
+  return fl == M3LB_FLUID || fl == M3LB_REGWALL || fl == M3LB_SPECNODE;
+}
+
+/* MirrorTable<L> (Specular.hpp:104-121): m[a][i] is the direction of c_i with
+ * component a negated. */
+static void build_mirror(S* s) {
+  const Lat* l = &s->c.l;
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < l->Q; ++i) {
+      int c[3] = {cvel(l, i, 0), cvel(l, i, 1), cvel(l, i, 2)};
+      c[a] = -c[a];
+      s->mirror[a][i] = i;
+      for (int j = 0; j < l->Q; ++j)
+        if (cvel(l, j, 0) == c[0] && cvel(l, j, 1) == c[1] && cvel(l, j, 2) == c[2]) {
+          s->mirror[a][i] = j;
+          break;
+        }
+    }
+}
+
+/* face_sign (Specular.hpp:253-258): outward sign of the face mask on axis a */
+static inline int face_sign(uint8_t faces, int a) {
+  return (faces & (1u << (2 * a))) ? 1 : ((faces & (1u << (2 * a + 1))) ? -1 : 0);
+}
+
+/* mirror_unknowns_faces (Specular.hpp:274-289): every direction pointing into
+ * the domain on a masked axis takes the value of its image through the
+ * composition of those axis mirrors. */
+static void mirror_unknowns_faces(const S* s, REAL* f, uint8_t faces) {
+  REAL h[27];
+  for (int i = 0; i < s->Q; ++i) h[i] = f[i];
+  for (int i = 0; i < s->Q; ++i) {
+    int j = i, unknown = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int sg = face_sign(faces, a);
+      if (sg != 0 && cvel(&s->c.l, i, a) * sg < 0) {
+        j = s->mirror[a][j];
+        unknown = 1;
+      }
+    }
+    if (unknown) f[i] = h[j];
+  }
 }
 
 static int is_interior(const Dom* d, int64_t px, int64_t py, int64_t pz) {
@@ -622,6 +695,13 @@ void* CORE(create)(const m3lb_config* cfg, int streaming) {
   if (streaming) s->f2 = (REAL*)calloc((size_t)(s->d.np * s->Q), sizeof(REAL));
   s->active = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->d.np > 0 ? s->d.np : 1));
   if (cfg->forcing == M3LB_FORCE_FIELD) s->ff = (REAL*)calloc((size_t)(3 * s->d.np), sizeof(REAL));
+  s->fd_corners = 1; /* fd_corners_ = has_shear_omega<Collision> (FluidSolver.hpp:1287) */
+  s->force_bc = 1;   /* force_bc_ (:1288) */
+  s->out_order = 1;  /* out_order_ (:1291) */
+  s->outflow_rho = R(1);
+  s->un_scale = R(1);
+  s->walls = (int64_t*)malloc(sizeof(int64_t) * (size_t)(s->d.np > 0 ? s->d.np : 1));
+  build_mirror(s);
   for (int64_t n = 0; n < s->d.np; ++n) {
     int64_t px, py, pz;
     coords(&s->d, n, &px, &py, &pz);
@@ -635,6 +715,10 @@ void CORE(destroy)(void* p) {
   S* s = (S*)p;
   if (!s) return;
   free(s->flags); free(s->f); free(s->f2); free(s->active); free(s->ff);
+  free(s->bc_nrm); free(s->bc_tag); free(s->bc_unk); free(s->bc_ext); free(s->bc_rho);
+  free(s->wtab); free(s->walls);
+  free(s->bc_don); free(s->bc_onrm); free(s->wtab0); free(s->wall_un);
+  free(s->spec_nrm); free(s->spec_faces);
   free(s);
 }
 
@@ -689,7 +773,7 @@ static void init_common(S* s, int field, REAL rho0, REAL ux0, REAL uy0, REAL uz0
   else
     seed_all(&s->c, &s->pt, rho0, R(0), R(0), R(0), rest);
   for (int64_t n = 0; n < s->d.np; ++n) {
-    if (s->flags[n] == M3LB_FLUID) {
+    if (fluid_like(s->flags[n])) {
       if (field)
         seed_all(&s->c, &s->pt, R(rho[n]), R(ux[n]), R(uy[n]), R(uz[n]), &vals[n * Q]);
       else
@@ -739,6 +823,347 @@ void CORE(storage)(void* p, double* out) {
   for (int64_t k = 0; k < s->d.np * s->Q; ++k) out[k] = (double)s->f[k];
 }
 
+/* ---------------------------------------------------------------------------
+ * Regularised velocity walls (Latt et al. 2008), src/boundary/Regularized.hpp
+ * with FluidSolver.hpp:372-460 (setup), 663-710 (corner density),
+ * 793-1020 (the wall path of run_step) and 1076-1185 (macro_kernel):
+ * velocity walls on straight faces and corners, and the three outflow codes.
+ * ------------------------------------------------------------------------- */
+#define NRM_CORNER 7
+#define NRM_OUT_XP 8
+#define NRM_OUT_FREE 9
+#define NRM_OUT_EQ 10
+
+static void normal_of(int code, int n[3]) {
+  n[0] = n[1] = n[2] = 0;
+  switch (code) {
+    case 1: n[0] = 1; break;
+    case 2: n[0] = -1; break;
+    case 3: n[1] = 1; break;
+    case 4: n[1] = -1; break;
+    case 5: n[2] = 1; break;
+    case 6: n[2] = -1; break;
+    case 8: n[0] = 1; break; /* NrmOutXp: outward normal, for the unknown set */
+    default: break;
+  }
+}
+
+/* Regularized::normal_velocity: the density closure inverted, rho known */
+static REAL bc_normal_velocity(const Lat* l, const REAL* f, const int nrm[3], REAL rho) {
+  REAL out = R(0), tang = R(0);
+  for (int i = 0; i < l->Q; ++i) {
+    const int cn = cvel(l, i, 0) * nrm[0] + cvel(l, i, 1) * nrm[1] + cvel(l, i, 2) * nrm[2];
+    if (cn > 0) out += f[i];
+    else if (cn == 0) tang += f[i];
+  }
+  return (R(2) * out + tang) / rho - R(1);
+}
+
+/* The wall velocity of node n: the per-node field when one is set
+ * (set_wall_velocity_field), else its table entry (FluidSolver.hpp:686-689). */
+static void wall_vel(const S* s, int64_t n, REAL uw[3]) {
+  if (s->wall_un) {
+    for (int a = 0; a < 3; ++a) uw[a] = s->un_scale * s->wall_un[a * s->d.np + n];
+    return;
+  }
+  const double* w = &s->wtab[4 * (int64_t)s->bc_tag[n]];
+  uw[0] = R(w[0]); uw[1] = R(w[1]); uw[2] = R(w[2]);
+}
+
+/* EquilibriumOf<Collision>: the operator's own equilibrium for BGK and TRT,
+ * the product form for the moment operators (Regularized.hpp:14-19,
+ * Equilibrium.hpp:130-136). */
+static int wall_product(const Coll* c) { return c->op != M3LB_OP_BGK && c->op != M3LB_OP_TRT; }
+
+/* Regularized::density */
+static REAL bc_density(const Lat* l, const REAL* f, const int nrm[3], const REAL u[3]) {
+  REAL out = R(0), tang = R(0);
+  for (int i = 0; i < l->Q; ++i) {
+    const int cn = cvel(l, i, 0) * nrm[0] + cvel(l, i, 1) * nrm[1] + cvel(l, i, 2) * nrm[2];
+    if (cn > 0) out += f[i];
+    else if (cn == 0) tang += f[i];
+  }
+  const REAL uperp = u[0] * R(nrm[0]) + u[1] * R(nrm[1]) + u[2] * R(nrm[2]);
+  return (R(2) * out + tang) / (R(1) + uperp);
+}
+
+static inline REAL qpi(const Lat* l, int i, const REAL Pi[6]) {
+  const REAL cs2v = CS2;
+  const REAL cx = R(cvel(l, i, 0)), cy = R(cvel(l, i, 1)), cz = R(cvel(l, i, 2));
+  return (cx * cx - cs2v) * Pi[0] + (cy * cy - cs2v) * Pi[1] +
+         ((l->D == 3) ? (cz * cz - cs2v) * Pi[2] : R(0)) +
+         R(2) * (cx * cy * Pi[3] + ((l->D == 3) ? (cx * cz * Pi[4] + cy * cz * Pi[5]) : R(0)));
+}
+
+/* Regularized::apply_with_stress */
+static void bc_apply_with_stress(const Coll* c, REAL* f, REAL rho, const REAL u[3],
+                                 const REAL Pi[6]) {
+  const Lat* l = &c->l;
+  const REAL cs2v = CS2, cs4v = cs2v * cs2v;
+  const int pr = wall_product(c);
+  for (int i = 0; i < l->Q; ++i)
+    f[i] = eq(l, pr, i, rho, u[0], u[1], u[2]) + weight(l, i) / (R(2) * cs4v) * qpi(l, i, Pi);
+}
+
+/* Regularized::stress_from_gradient */
+static void bc_stress_from_gradient(REAL rho, REAL omega, const REAL g[3][3], REAL Pi[6]) {
+  const REAL k = -R(2) * CS2 * rho / omega;
+  Pi[0] = k * g[0][0];
+  Pi[1] = k * g[1][1];
+  Pi[2] = k * g[2][2];
+  Pi[3] = k * R(0.5) * (g[0][1] + g[1][0]);
+  Pi[4] = k * R(0.5) * (g[0][2] + g[2][0]);
+  Pi[5] = k * R(0.5) * (g[1][2] + g[2][1]);
+}
+
+/* Regularized::apply: unknown directions from the bounced non-equilibrium
+ * part (with the Guo odd part when a force is given), then every direction
+ * rebuilt from (rho, u, Pi^(1)). */
+static void bc_apply(const Coll* c, REAL* f, REAL rho, const REAL u[3], uint32_t unk,
+                     const REAL* Fv) {
+  const Lat* l = &c->l;
+  const REAL cs2v = CS2, cs4v = cs2v * cs2v;
+  const int pr = wall_product(c);
+  REAL feq[27];
+  for (int i = 0; i < l->Q; ++i) feq[i] = eq(l, pr, i, rho, u[0], u[1], u[2]);
+  const REAL hodd = R(0.5) / cs2v;
+  for (int i = 0; i < l->Q; ++i) {
+    if (!(unk & (1u << i))) continue;
+    const REAL cF = Fv ? (R(cvel(l, i, 0)) * Fv[0] + R(cvel(l, i, 1)) * Fv[1] +
+                          R(cvel(l, i, 2)) * Fv[2])
+                       : R(0);
+    const REAL odd = weight(l, i) * hodd * cF;
+    f[i] = (unk & (1u << opp(i))) ? feq[i] - odd
+                                  : feq[i] + (f[opp(i)] - feq[opp(i)]) - R(2) * odd;
+  }
+  REAL Pi[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < l->Q; ++i) {
+    const REAL d = f[i] - feq[i];
+    const REAL cx = R(cvel(l, i, 0)), cy = R(cvel(l, i, 1)), cz = R(cvel(l, i, 2));
+    Pi[0] += d * cx * cx; Pi[1] += d * cy * cy; Pi[2] += d * cz * cz;
+    Pi[3] += d * cx * cy; Pi[4] += d * cx * cz; Pi[5] += d * cy * cz;
+  }
+  for (int i = 0; i < l->Q; ++i) {
+    const REAL cx = R(cvel(l, i, 0)), cy = R(cvel(l, i, 1)), cz = R(cvel(l, i, 2));
+    const REAL cF = Fv ? (cx * Fv[0] + cy * Fv[1] + cz * Fv[2]) : R(0);
+    f[i] = feq[i] - weight(l, i) * hodd * cF + weight(l, i) / (R(2) * cs4v) * qpi(l, i, Pi);
+  }
+}
+
+/* Incoming set of node n at parity P, unshifted to f. */
+static void load_raw(const S* s, int P, int64_t n, REAL* f) {
+  int64_t j[27];
+  fill_neighbours(&s->d, &s->c.l, 1, 2, n, j);
+  for (int i = 0; i < s->Q; ++i) f[i] = ep_load(s, P, j, n, i);
+  if (s->c.shifted)
+    for (int i = 0; i < s->Q; ++i) f[i] += weight(&s->c.l, i);
+}
+
+static inline int64_t interior_id(const Dom* d, int64_t x, int64_t y, int64_t z) {
+  return ((z + d->hz) * d->sy + (y + d->hy)) * d->sx + x + d->hx;
+}
+
+/* corner_density<P> (FluidSolver.hpp:663-710): rho at every straight-wall
+ * node from its incoming set, then each corner extrapolated along the axis
+ * chosen at setup (2 rho(n1) - rho(n2)), or 1 without a stencil. */
+static void corner_density(S* s, int P) {
+  REAL f[27];
+  for (int64_t k = 0; k < s->n_walls; ++k) {
+    const int64_t n = s->walls[k];
+    const int code = s->bc_nrm[n];
+    if (code == NRM_CORNER || code == 0) continue;
+    load_raw(s, P, n, f);
+    int nrm[3];
+    normal_of(code, nrm);
+    REAL uw[3];
+    wall_vel(s, n, uw);
+    const REAL trho = R(s->wtab[4 * (int64_t)s->bc_tag[n] + 3]);
+    s->bc_rho[n] = (code == NRM_OUT_XP || code == NRM_OUT_EQ) ? trho
+                 : (code == NRM_OUT_FREE) ? s->outflow_rho
+                                          : bc_density(&s->c.l, f, nrm, uw);
+  }
+  static const int dirs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  for (int64_t k = 0; k < s->n_walls; ++k) {
+    const int64_t n = s->walls[k];
+    if (s->bc_nrm[n] != NRM_CORNER) continue;
+    const int e = s->bc_ext[n];
+    if (e == 0) { s->bc_rho[n] = R(1); continue; }
+    int64_t px, py, pz;
+    coords(&s->d, n, &px, &py, &pz);
+    const int64_t x = px - s->d.hx, y = py - s->d.hy, z = pz - s->d.hz;
+    const int* v = dirs[e - 1];
+    const int64_t n1 = interior_id(&s->d, x + v[0], y + v[1], z + v[2]);
+    const int64_t n2 = interior_id(&s->d, x + 2 * v[0], y + 2 * v[1], z + 2 * v[2]);
+    s->bc_rho[n] = R(2) * s->bc_rho[n1] - s->bc_rho[n2];
+  }
+}
+
+/* velocity at node m for the finite-difference corner stress: the wall
+ * velocity of a wall node, else from m's incoming set. */
+static void fd_vel(const S* s, int P, int64_t m, REAL out[3]) {
+  if (s->flags[m] == M3LB_REGWALL) {
+    const double* w = &s->wtab[4 * (int64_t)s->bc_tag[m]];
+    out[0] = R(w[0]); out[1] = R(w[1]); out[2] = R(w[2]);
+    return;
+  }
+  REAL g[27];
+  load_raw(s, P, m, g);
+  REAL sm = 0, mx = 0, my = 0, mz = 0;
+  for (int i = 0; i < s->Q; ++i) {
+    sm += g[i];
+    mx += g[i] * R(cvel(&s->c.l, i, 0));
+    my += g[i] * R(cvel(&s->c.l, i, 1));
+    mz += g[i] * R(cvel(&s->c.l, i, 2));
+  }
+  const REAL ir = R(1) / sm;
+  out[0] = mx * ir; out[1] = my * ir; out[2] = mz * ir;
+}
+
+/* The mean velocity of node m's incoming set (the upstream and donor
+ * velocities of the outflow codes, FluidSolver.hpp:935-950).  guard: return 0
+ * for a non-positive sum instead of dividing. */
+static void mean_velocity(const S* s, int P, int64_t m, REAL* sm_out, REAL v[3], int guard) {
+  REAL g[27];
+  load_raw(s, P, m, g);
+  REAL sm = R(0);
+  v[0] = v[1] = v[2] = R(0);
+  for (int i = 0; i < s->Q; ++i) {
+    sm += g[i];
+    for (int a = 0; a < 3; ++a) v[a] += g[i] * R(cvel(&s->c.l, i, a));
+  }
+  const REAL ir = guard ? ((sm > R(0)) ? R(1) / sm : R(0)) : R(1) / sm;
+  for (int a = 0; a < 3; ++a) v[a] *= ir;
+  if (sm_out) *sm_out = sm;
+}
+
+/* NrmOutXp's velocity (FluidSolver.hpp:912-951, 1143-1180): rho imposed, the
+ * normal component from the inverted closure on the node's own set f, the
+ * tangential ones zero-gradient (order 1) or linearly extrapolated (order 2)
+ * from the upstream neighbours along -x. */
+static void outflow_xp_velocity(const S* s, int P, int64_t n, const REAL* f, REAL rw, REAL ur[3]) {
+  int64_t px, py, pz;
+  coords(&s->d, n, &px, &py, &pz);
+  const int64_t bx = px - s->d.hx, by = py - s->d.hy, bz = pz - s->d.hz;
+  REAL ut[3];
+  mean_velocity(s, P, interior_id(&s->d, bx - 1, by, bz), NULL, ut, 0);
+  if (s->out_order >= 2) {
+    REAL v2[3];
+    mean_velocity(s, P, interior_id(&s->d, bx - 2, by, bz), NULL, v2, 0);
+    for (int a = 0; a < 3; ++a) ut[a] = R(2) * ut[a] - v2[a];
+  }
+  const int nrm[3] = {1, 0, 0};
+  const REAL un = bc_normal_velocity(&s->c.l, f, nrm, rw);
+  REAL utn = R(0);
+  for (int a = 0; a < 3; ++a) utn += ut[a] * R(nrm[a]);
+  for (int a = 0; a < 3; ++a) ur[a] = ut[a] + (un - utn) * R(nrm[a]);
+}
+
+/* The RegWall node update of run_step (FluidSolver.hpp:793-1033): velocity
+ * walls, corners and the outflow codes; rebuild, collide, store (the
+ * population-extrapolation and equilibrium outlets store without colliding). */
+static void wall_update(S* s, int P, int64_t n) {
+  const Lat* l = &s->c.l;
+  const int Q = s->Q;
+  int64_t j[27];
+  REAL f[27];
+  fill_neighbours(&s->d, l, 1, 2, n, j);
+  f[0] = F(s, n, 0);
+  for (int i = 1; i < Q; i += 2) {
+    if (P == 0) { f[i] = F(s, n, i); f[i + 1] = F(s, j[i], i + 1); }
+    else        { f[i] = F(s, n, i + 1); f[i + 1] = F(s, j[i], i); }
+  }
+  if (s->c.shifted)
+    for (int i = 0; i < Q; ++i) f[i] += weight(l, i);
+  const int code = s->bc_nrm[n];
+  int nrm[3];
+  normal_of(code, nrm);
+  const int tg = s->bc_tag[n];
+  REAL uw[3];
+  wall_vel(s, n, uw);
+  Coll cn;
+  const Coll* cp = coll_at(s, n, &cn);
+  REAL Fv[3] = {R(0), R(0), R(0)};
+  if (cp->forcing && s->force_bc) { Fv[0] = cp->F[0]; Fv[1] = cp->F[1]; Fv[2] = cp->F[2]; }
+  REAL ur[3] = {uw[0], uw[1], uw[2]};
+  REAL rw;
+  int collide_it = 1;
+  if (code == NRM_OUT_FREE) {
+    /* the donor's distribution rescaled to the imposed density (:843-880) */
+    REAL g[27];
+    load_raw(s, P, s->bc_don[n], g);
+    REAL sm = R(0);
+    for (int i = 0; i < Q; ++i) sm += g[i];
+    if (sm > R(0)) {
+      const REAL sc = s->outflow_rho / sm;
+      for (int i = 0; i < Q; ++i) f[i] = g[i] * sc;
+    }
+    collide_it = 0;
+  } else if (code == NRM_OUT_EQ) {
+    /* equilibrium at the imposed rho and the upstream velocity (:881-911) */
+    const REAL rw2 = R(s->wtab[4 * (int64_t)tg + 3]);
+    int64_t px, py, pz;
+    coords(&s->d, n, &px, &py, &pz);
+    REAL mu[3];
+    mean_velocity(s, P, interior_id(&s->d, px - s->d.hx - 1, py - s->d.hy, pz - s->d.hz), NULL,
+                  mu, 1);
+    for (int i = 0; i < Q; ++i) f[i] = eq(l, wall_product(&s->c), i, rw2, mu[0], mu[1], mu[2]);
+    collide_it = 0;
+  } else if (code == NRM_OUT_XP) {
+    rw = R(s->wtab[4 * (int64_t)tg + 3]);
+    outflow_xp_velocity(s, P, n, f, rw, ur);
+  } else {
+    rw = (code == NRM_CORNER) ? s->bc_rho[n] : bc_density(l, f, nrm, ur);
+  }
+  if (collide_it) {
+    if (s->fd_corners && code == NRM_CORNER) {
+      int64_t px, py, pz;
+      coords(&s->d, n, &px, &py, &pz);
+      const int64_t ic[3] = {px - s->d.hx, py - s->d.hy, pz - s->d.hz};
+      const int64_t nn[3] = {s->d.nx, s->d.ny, s->d.nz};
+      REAL grad[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int a = 0; a < l->D; ++a) {
+        if (nn[a] < 3) continue;
+        int dir = 0;
+        if (!s->d.per[a]) {
+          if (ic[a] == 0) dir = 1;
+          else if (ic[a] == nn[a] - 1) dir = -1;
+        }
+        int64_t c1[3] = {ic[0], ic[1], ic[2]}, c2[3] = {ic[0], ic[1], ic[2]};
+        REAL u1[3], u2[3];
+        if (dir != 0) {
+          c1[a] += dir; c2[a] += 2 * dir;
+          fd_vel(s, P, interior_id(&s->d, c1[0], c1[1], c1[2]), u1);
+          fd_vel(s, P, interior_id(&s->d, c2[0], c2[1], c2[2]), u2);
+          for (int b = 0; b < l->D; ++b)
+            grad[a][b] = R(dir) * (R(-1.5) * uw[b] + R(2) * u1[b] - R(0.5) * u2[b]);
+        } else {
+          c1[a] = (ic[a] + 1) % nn[a];
+          c2[a] = (ic[a] - 1 + nn[a]) % nn[a];
+          fd_vel(s, P, interior_id(&s->d, c1[0], c1[1], c1[2]), u1);
+          fd_vel(s, P, interior_id(&s->d, c2[0], c2[1], c2[2]), u2);
+          for (int b = 0; b < l->D; ++b) grad[a][b] = R(0.5) * (u1[b] - u2[b]);
+        }
+      }
+      REAL Pi[6];
+      bc_stress_from_gradient(rw, s->c.omega, grad, Pi);
+      bc_apply_with_stress(&s->c, f, rw, ur, Pi);
+    } else {
+      bc_apply(&s->c, f, rw, ur, s->bc_unk[n], Fv);
+    }
+  }
+  if (s->c.shifted)
+    for (int i = 0; i < Q; ++i) f[i] -= weight(l, i);
+  if (collide_it) {
+    const Macro m = macroscopic(cp, f);
+    collide(cp, &s->pt, f, &m);
+  }
+  F(s, n, 0) = f[0];
+  for (int i = 1; i < Q; i += 2) {
+    if (P == 0) { F(s, j[i], i + 1) = f[i]; F(s, n, i) = f[i + 1]; }
+    else        { F(s, j[i], i) = f[i]; F(s, n, i + 1) = f[i + 1]; }
+  }
+}
+
 /* run_step<P>, FluidSolver.hpp:712-1038, Fluid and Solid paths only; one
  * contiguous range of the active list. */
 typedef struct {
@@ -759,6 +1184,27 @@ static void* step_range(void* arg) {
     const uint8_t flag = s->flags[n];
     if (flag == M3LB_EXCLUDED) continue;
     if (s->streaming == 0) {
+      if (flag == M3LB_SPECWALL) {
+        /* specular ghost (:745-756): the incoming set permuted about the
+         * wall normal's axis and stored back, no collision */
+        int nv[3];
+        normal_of(s->spec_nrm[n], nv);
+        const int ax = nv[0] ? 0 : (nv[1] ? 1 : 2);
+        fill_neighbours(&s->d, &s->c.l, 1, 2, n, j);
+        REAL fin[27];
+        fin[0] = F(s, n, 0);
+        for (int i = 1; i < Q; i += 2) {
+          if (P == 0) { fin[i] = F(s, n, i); fin[i + 1] = F(s, j[i], i + 1); }
+          else        { fin[i] = F(s, n, i + 1); fin[i + 1] = F(s, j[i], i); }
+        }
+        for (int i = 0; i < Q; ++i) f[i] = fin[s->mirror[ax][i]];
+        F(s, n, 0) = f[0];
+        for (int i = 1; i < Q; i += 2) {
+          if (P == 0) { F(s, j[i], i + 1) = f[i]; F(s, n, i) = f[i + 1]; }
+          else        { F(s, j[i], i) = f[i]; F(s, n, i + 1) = f[i + 1]; }
+        }
+        continue;
+      }
       if (flag == M3LB_SOLID) continue; /* implicit bounce-back, :759-761 */
       fill_neighbours(&s->d, &s->c.l, 1, 2, n, j);
       /* load_rest / load_pair, EsotericPull.hpp:213-223 */
@@ -767,6 +1213,8 @@ static void* step_range(void* arg) {
         if (P == 0) { f[i] = F(s, n, i); f[i + 1] = F(s, j[i], i + 1); }
         else        { f[i] = F(s, n, i + 1); f[i + 1] = F(s, j[i], i); }
       }
+      /* on-node specular (:782-793): mirror the unknowns, then collide */
+      if (flag == M3LB_SPECNODE) mirror_unknowns_faces(s, f, s->spec_faces[n]);
       Coll cn;
       const Coll* cp = coll_at(s, n, &cn);
       const Macro m = macroscopic(cp, f);
@@ -802,6 +1250,15 @@ static void* step_range(void* arg) {
 static void run_step(S* s) {
   const int P = (int)(s->t % 2);
   const int nt = s->nthreads;
+  if (s->n_walls && s->streaming == 0) {
+    /* FluidSolver::step runs the corner-density pass first (:583).  Wall
+     * nodes are updated before the fluid nodes: every node reads and writes
+     * only its own slots, so the order changes nothing but what a corner's
+     * finite-difference stencil reads from a fluid neighbour (its incoming
+     * set, before this step). */
+    if (s->has_corners) corner_density(s, P);
+    for (int64_t k = 0; k < s->n_walls; ++k) wall_update(s, P, s->walls[k]);
+  }
   if (nt <= 1 || s->n_active < 4096) {
     Range r = {s, P, 0, s->n_active};
     step_range(&r);
@@ -838,6 +1295,219 @@ double CORE(time_steps)(void* p, int nsteps) {
 }
 
 uint64_t CORE(timestep)(const void* p) { return ((const S*)p)->t; }
+
+/* set_regularized_walls (FluidSolver.hpp:372-530): normals (0..10) and
+ * tags per INTERIOR node (x fastest), the deduplicated wall-state table
+ * (ux, uy, uz, rho per entry).  Marks the nodes RegWall, picks the corner
+ * extrapolation axis, the unknown-direction masks and the outflow donors
+ * exactly as the reference does, from the flags as they are now.  Returns 0,
+ * or -1 for a bad argument (then nothing changes). */
+int CORE(set_regularized_walls)(void* p, const uint8_t* normals, const uint16_t* tags,
+                                const double* table, int ntab) {
+  S* s = (S*)p;
+  const Dom* d = &s->d;
+  const int64_t np = d->np;
+  const int64_t ni = d->nx * d->ny * d->nz;
+  if (s->streaming != 0 || ntab < 1) return -1;
+  for (int64_t k = 0; k < ni; ++k)
+    if (normals[k] > NRM_OUT_EQ || (normals[k] && tags[k] >= ntab)) return -1;
+  if (!s->bc_nrm) {
+    s->bc_nrm = (uint8_t*)calloc((size_t)np, 1);
+    s->bc_tag = (uint16_t*)calloc((size_t)np, sizeof(uint16_t));
+    s->bc_unk = (uint32_t*)calloc((size_t)np, sizeof(uint32_t));
+    s->bc_ext = (uint8_t*)calloc((size_t)np, 1);
+    s->bc_rho = (REAL*)calloc((size_t)np, sizeof(REAL));
+    s->bc_don = (int64_t*)calloc((size_t)np, sizeof(int64_t));
+    s->bc_onrm = (uint8_t*)calloc((size_t)np, 1);
+  }
+  free(s->wtab);
+  free(s->wtab0);
+  s->wtab = (double*)malloc(sizeof(double) * 4 * (size_t)ntab);
+  s->wtab0 = (double*)malloc(sizeof(double) * 4 * (size_t)ntab);
+  /* the table holds Reals (View2D<Real> wall_u_) */
+  for (int64_t k = 0; k < 4 * (int64_t)ntab; ++k) s->wtab[k] = s->wtab0[k] = (double)R(table[k]);
+  s->ntab = ntab;
+  for (int64_t n = 0; n < np; ++n) { s->bc_nrm[n] = 0; s->bc_tag[n] = 0; s->bc_ext[n] = 0; s->bc_unk[n] = 0; }
+  int64_t k = 0;
+  for (int64_t z = 0; z < d->nz; ++z)
+    for (int64_t y = 0; y < d->ny; ++y)
+      for (int64_t x = 0; x < d->nx; ++x, ++k) {
+        if (normals[k] == 0) continue;
+        const int64_t n = interior_id(d, x, y, z);
+        s->bc_nrm[n] = normals[k];
+        s->bc_tag[n] = tags[k];
+        s->flags[n] = M3LB_REGWALL;
+      }
+  static const int dirs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  s->has_corners = 0;
+  for (int64_t z = 0; z < d->nz; ++z)
+    for (int64_t y = 0; y < d->ny; ++y)
+      for (int64_t x = 0; x < d->nx; ++x) {
+        const int64_t n = interior_id(d, x, y, z);
+        if (s->bc_nrm[n] != NRM_CORNER) continue;
+        s->has_corners = 1;
+        for (int q = 0; q < (d->nz > 1 ? 6 : 4); ++q) {
+          const int64_t x1 = x + dirs[q][0], y1 = y + dirs[q][1], z1 = z + dirs[q][2];
+          const int64_t x2 = x + 2 * dirs[q][0], y2 = y + 2 * dirs[q][1], z2 = z + 2 * dirs[q][2];
+          if (x1 < 0 || y1 < 0 || z1 < 0 || x2 < 0 || y2 < 0 || z2 < 0) continue;
+          if (x1 >= d->nx || y1 >= d->ny || z1 >= d->nz) continue;
+          if (x2 >= d->nx || y2 >= d->ny || z2 >= d->nz) continue;
+          const int a = s->bc_nrm[interior_id(d, x1, y1, z1)];
+          const int b = s->bc_nrm[interior_id(d, x2, y2, z2)];
+          if (a != 0 && a != NRM_CORNER && b != 0 && b != NRM_CORNER) {
+            s->bc_ext[n] = (uint8_t)(q + 1);
+            break;
+          }
+        }
+      }
+  /* unknown directions: the source p - c_i outside a non-periodic extent, or
+   * an Excluded or Solid node */
+  const int64_t next[3] = {d->nx, d->ny, d->nz};
+  for (int64_t z = 0; z < d->nz; ++z)
+    for (int64_t y = 0; y < d->ny; ++y)
+      for (int64_t x = 0; x < d->nx; ++x) {
+        const int64_t n = interior_id(d, x, y, z);
+        if (s->bc_nrm[n] == 0) continue;
+        uint32_t m = 0;
+        for (int i = 0; i < s->Q; ++i) {
+          int64_t src[3] = {x - cvel(&s->c.l, i, 0), y - cvel(&s->c.l, i, 1), z - cvel(&s->c.l, i, 2)};
+          int outside = 0;
+          for (int a = 0; a < 3; ++a) {
+            if (d->per[a]) src[a] = (src[a] + next[a]) % next[a];
+            else if (src[a] < 0 || src[a] >= next[a]) outside = 1;
+          }
+          if (!outside) {
+            const uint8_t sf = s->flags[interior_id(d, src[0], src[1], src[2])];
+            if (sf == M3LB_EXCLUDED || sf == M3LB_SOLID) outside = 1;
+          }
+          if (outside) m |= (1u << i);
+        }
+        s->bc_unk[n] = m;
+      }
+  /* NrmOutFree donors (:475-520): the first axis step that leaves the fluid
+   * (outside or Solid) whose opposite neighbour is Fluid; a node without one
+   * points at itself */
+  {
+    static const int dd[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    static const uint8_t codes[6] = {2, 1, 4, 3, 6, 5};
+    for (int64_t n = 0; n < np; ++n) { s->bc_don[n] = n; s->bc_onrm[n] = 0; }
+    for (int64_t z = 0; z < d->nz; ++z)
+      for (int64_t y = 0; y < d->ny; ++y)
+        for (int64_t x = 0; x < d->nx; ++x) {
+          const int64_t n = interior_id(d, x, y, z);
+          if (s->bc_nrm[n] != NRM_OUT_FREE) continue;
+          for (int q = 0; q < 6; ++q) {
+            const int64_t qx = x + dd[q][0], qy = y + dd[q][1], qz = z + dd[q][2];
+            const int leaves = (qx < 0 || qx >= d->nx || qy < 0 || qy >= d->ny || qz < 0 ||
+                                qz >= d->nz) ||
+                               s->flags[interior_id(d, qx, qy, qz)] == M3LB_SOLID;
+            if (!leaves) continue;
+            const int o = q ^ 1;
+            const int64_t rx = x + dd[o][0], ry = y + dd[o][1], rz = z + dd[o][2];
+            if (rx < 0 || rx >= d->nx || ry < 0 || ry >= d->ny || rz < 0 || rz >= d->nz) continue;
+            const int64_t m = interior_id(d, rx, ry, rz);
+            if (s->flags[m] != M3LB_FLUID) continue;
+            s->bc_don[n] = m;
+            s->bc_onrm[n] = codes[q];
+            break;
+          }
+        }
+  }
+  rebuild_lists(s);
+  return 0;
+}
+
+/* set_fd_corners (FluidSolver.hpp:272) */
+void CORE(set_fd_corners)(void* p, int on) { ((S*)p)->fd_corners = on ? 1 : 0; }
+/* set_force_correction (:1311), set_outflow_order (:1324), set_outflow_density (:1330) */
+void CORE(set_force_correction)(void* p, int on) { ((S*)p)->force_bc = on ? 1 : 0; }
+void CORE(set_outflow_order)(void* p, int k) { ((S*)p)->out_order = k; }
+void CORE(set_outflow_density)(void* p, double r) { ((S*)p)->outflow_rho = R(r); }
+
+/* set_wall_velocity_scale (:303-312): the velocities of the table as given,
+ * times sc (in Real) */
+void CORE(set_wall_velocity_scale)(void* p, double sc) {
+  S* s = (S*)p;
+  for (int k = 0; k < s->ntab; ++k)
+    for (int a = 0; a < 3; ++a) s->wtab[4 * k + a] = (double)(R(s->wtab0[4 * k + a]) * R(sc));
+}
+/* set_wall_density (:330-335): slot 3 of every state, and of the table as given */
+void CORE(set_wall_density)(void* p, double r) {
+  S* s = (S*)p;
+  for (int k = 0; k < s->ntab; ++k) s->wtab[4 * k + 3] = s->wtab0[4 * k + 3] = (double)R(r);
+}
+/* set_wall_velocity_field (:353-367): one velocity per interior node (x
+ * fastest), read instead of the table from now on; its scale restarts at 1 */
+void CORE(set_wall_velocity_field)(void* p, const double* ux, const double* uy, const double* uz) {
+  S* s = (S*)p;
+  const Dom* d = &s->d;
+  if (!s->wall_un) s->wall_un = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)d->np);
+  for (int64_t n = 0; n < 3 * d->np; ++n) s->wall_un[n] = R(0);
+  int64_t k = 0;
+  for (int64_t z = 0; z < d->nz; ++z)
+    for (int64_t y = 0; y < d->ny; ++y)
+      for (int64_t x = 0; x < d->nx; ++x, ++k) {
+        const int64_t n = interior_id(d, x, y, z);
+        s->wall_un[n] = R(ux[k]);
+        s->wall_un[d->np + n] = R(uy[k]);
+        s->wall_un[2 * d->np + n] = R(uz[k]);
+      }
+  s->un_scale = R(1);
+}
+void CORE(set_wall_velocity_field_scale)(void* p, double sc) { ((S*)p)->un_scale = R(sc); }
+
+/* set_specular_walls (:193-216): axis normals 1..6 per interior node mark
+ * ghost specular cells; every other cell's normal is cleared.  Returns -1
+ * (and changes nothing) for a code above 6. */
+int CORE(set_specular_walls)(void* p, const uint8_t* normals) {
+  S* s = (S*)p;
+  const Dom* d = &s->d;
+  const int64_t ni = d->nx * d->ny * d->nz;
+  if (s->streaming != 0) return -1;
+  for (int64_t k = 0; k < ni; ++k)
+    if (normals[k] > 6) return -1;
+  if (!s->spec_nrm) s->spec_nrm = (uint8_t*)calloc((size_t)d->np, 1);
+  for (int64_t n = 0; n < d->np; ++n) s->spec_nrm[n] = 0;
+  int64_t k = 0;
+  for (int64_t z = 0; z < d->nz; ++z)
+    for (int64_t y = 0; y < d->ny; ++y)
+      for (int64_t x = 0; x < d->nx; ++x, ++k) {
+        if (!normals[k]) continue;
+        const int64_t n = interior_id(d, x, y, z);
+        s->spec_nrm[n] = normals[k];
+        s->flags[n] = M3LB_SPECWALL;
+      }
+  rebuild_lists(s);
+  return 0;
+}
+
+/* set_specular_nodes (:236-270): a face mask per interior node marks on-node
+ * specular nodes.  Returns -1 (and changes nothing) for bits above 0x3F or
+ * both faces of one axis. */
+int CORE(set_specular_nodes)(void* p, const uint8_t* faces) {
+  S* s = (S*)p;
+  const Dom* d = &s->d;
+  const int64_t ni = d->nx * d->ny * d->nz;
+  if (s->streaming != 0) return -1;
+  for (int64_t k = 0; k < ni; ++k) {
+    if (faces[k] > 0x3F) return -1;
+    for (int a = 0; a < 3; ++a)
+      if ((faces[k] & (1u << (2 * a))) && (faces[k] & (1u << (2 * a + 1)))) return -1;
+  }
+  if (!s->spec_faces) s->spec_faces = (uint8_t*)calloc((size_t)d->np, 1);
+  for (int64_t n = 0; n < d->np; ++n) s->spec_faces[n] = 0;
+  int64_t k = 0;
+  for (int64_t z = 0; z < d->nz; ++z)
+    for (int64_t y = 0; y < d->ny; ++y)
+      for (int64_t x = 0; x < d->nx; ++x, ++k) {
+        if (!faces[k]) continue;
+        const int64_t n = interior_id(d, x, y, z);
+        s->spec_faces[n] = faces[k];
+        s->flags[n] = M3LB_SPECNODE;
+      }
+  rebuild_lists(s);
+  return 0;
+}
 
 void CORE(set_force_field)(void* p, const double* fx, const double* fy, const double* fz) {
   S* s = (S*)p;
@@ -880,6 +1550,59 @@ void CORE(compute_macroscopic)(void* p, double* rho, double* ux, double* uy, dou
       Coll cn;
       const Macro m = macroscopic(coll_at(s, n, &cn), f);
       r = density(&s->c, &m); a = m.ux; b = m.uy; c = m.uz;
+    } else if (s->flags[n] == M3LB_SPECNODE) {
+      /* macro_kernel's SpecNode path (:1057-1077): raw moments of the
+       * mirrored incoming set */
+      load_node(s, n, f);
+      mirror_unknowns_faces(s, f, s->spec_faces[n]);
+      if (s->c.shifted)
+        for (int i = 0; i < s->Q; ++i) f[i] += weight(&s->c.l, i);
+      REAL sm = R(0), mx = R(0), my = R(0), mz = R(0);
+      for (int i = 0; i < s->Q; ++i) {
+        sm += f[i];
+        mx += f[i] * R(cvel(&s->c.l, i, 0));
+        my += f[i] * R(cvel(&s->c.l, i, 1));
+        mz += f[i] * R(cvel(&s->c.l, i, 2));
+      }
+      const REAL ir = R(1) / sm;
+      r = sm; a = mx * ir; b = my * ir; c = mz * ir;
+    } else if (s->flags[n] == M3LB_REGWALL) {
+      /* macro_kernel's RegWall path (:1078-1185): the imposed (or outflow)
+       * velocity and the closure (or imposed) density; a corner reports the
+       * last pass's rho */
+      const int P = (int)(s->t % 2);
+      const int code = s->bc_nrm[n];
+      const int tg = s->bc_tag[n];
+      REAL uw[3];
+      wall_vel(s, n, uw);
+      load_raw(s, P, n, f);
+      int nrm[3];
+      normal_of(code, nrm);
+      if (code == NRM_OUT_FREE) {
+        r = s->outflow_rho;
+        int onr[3];
+        normal_of(s->bc_onrm[n], onr);
+        REAL sm, m4[3];
+        mean_velocity(s, P, s->bc_don[n], &sm, m4, 1);
+        const REAL un = bc_normal_velocity(&s->c.l, f, onr, r);
+        REAL utn = R(0);
+        for (int k = 0; k < 3; ++k) utn += m4[k] * R(onr[k]);
+        for (int k = 0; k < 3; ++k) uw[k] = m4[k] + (un - utn) * R(onr[k]);
+      } else if (code == NRM_OUT_EQ) {
+        r = R(s->wtab[4 * (int64_t)tg + 3]);
+        int64_t px, py, pz;
+        coords(&s->d, n, &px, &py, &pz);
+        mean_velocity(s, P, interior_id(&s->d, px - s->d.hx - 1, py - s->d.hy, pz - s->d.hz),
+                      NULL, uw, 1);
+      } else if (code == NRM_OUT_XP) {
+        r = R(s->wtab[4 * (int64_t)tg + 3]);
+        REAL ur[3];
+        outflow_xp_velocity(s, P, n, f, r, ur);
+        uw[0] = ur[0]; uw[1] = ur[1]; uw[2] = ur[2];
+      } else {
+        r = (code == NRM_CORNER) ? s->bc_rho[n] : bc_density(&s->c.l, f, nrm, uw);
+      }
+      a = uw[0]; b = uw[1]; c = uw[2];
     }
     if (rho) rho[n] = (double)r;
     if (ux) ux[n] = (double)a;
@@ -893,8 +1616,10 @@ static double reduce(S* s, int axis) {
   REAL f[27];
   double sum = 0;
   for (int64_t n = 0; n < s->d.np; ++n) {
-    if (s->flags[n] != M3LB_FLUID) continue;
+    if (!fluid_like(s->flags[n])) continue;
     load_node(s, n, f);
+    /* the streamed unknowns of a SpecNode hold nothing until mirrored */
+    if (s->flags[n] == M3LB_SPECNODE) mirror_unknowns_faces(s, f, s->spec_faces[n]);
     Coll cn;
     const Macro m = macroscopic(coll_at(s, n, &cn), f);
     const REAL r = density(&s->c, &m);

@@ -21,6 +21,18 @@ double core64_time_steps(void* p, int nsteps);
 uint64_t core64_timestep(const void* p);
 void core64_set_force(void* p, double fx, double fy, double fz);
 void core64_set_force_field(void* p, const double* fx, const double* fy, const double* fz);
+int core64_set_regularized_walls(void* p, const uint8_t* normals, const uint16_t* tags,
+                                 const double* table, int ntab);
+void core64_set_fd_corners(void* p, int on);
+void core64_set_force_correction(void* p, int on);
+void core64_set_outflow_order(void* p, int k);
+void core64_set_outflow_density(void* p, double r);
+void core64_set_wall_velocity_scale(void* p, double sc);
+void core64_set_wall_density(void* p, double r);
+void core64_set_wall_velocity_field(void* p, const double* ux, const double* uy, const double* uz);
+void core64_set_wall_velocity_field_scale(void* p, double sc);
+int core64_set_specular_walls(void* p, const uint8_t* normals);
+int core64_set_specular_nodes(void* p, const uint8_t* faces);
 void core64_compute_macroscopic(void* p, double* rho, double* ux, double* uy, double* uz);
 double core64_total_mass(void* p);
 double core64_total_momentum(void* p, int axis);
@@ -48,6 +60,18 @@ double core32_time_steps(void* p, int nsteps);
 uint64_t core32_timestep(const void* p);
 void core32_set_force(void* p, double fx, double fy, double fz);
 void core32_set_force_field(void* p, const double* fx, const double* fy, const double* fz);
+int core32_set_regularized_walls(void* p, const uint8_t* normals, const uint16_t* tags,
+                                 const double* table, int ntab);
+void core32_set_fd_corners(void* p, int on);
+void core32_set_force_correction(void* p, int on);
+void core32_set_outflow_order(void* p, int k);
+void core32_set_outflow_density(void* p, double r);
+void core32_set_wall_velocity_scale(void* p, double sc);
+void core32_set_wall_density(void* p, double r);
+void core32_set_wall_velocity_field(void* p, const double* ux, const double* uy, const double* uz);
+void core32_set_wall_velocity_field_scale(void* p, double sc);
+int core32_set_specular_walls(void* p, const uint8_t* normals);
+int core32_set_specular_nodes(void* p, const uint8_t* faces);
 void core32_compute_macroscopic(void* p, double* rho, double* ux, double* uy, double* uz);
 double core32_total_mass(void* p);
 double core32_total_momentum(void* p, int axis);

@@ -13,6 +13,9 @@ Sources (all in the reference checkout):
   README.md, Poiseuille slip and L2 tables         validation/poiseuille.cpp
   results/H_forcing_convergence/forcing_conv.dat   validation/forcing_transverse.cpp -conv,
                                                    D2Q9 CM, both orientations
+  results/B_shear_layer/shear_{d2q9,d3q27}_cm.dat  validation/shear_layer.cpp, L=256 Re=3e4
+  results/C_cavity/cavity_re*_{d2q9,d3q27}_cm*.dat validation/cavity_cm.cpp, N=129 u_lid=0.02,
+                                                   17 Ghia stations and full centrelines
 """
 import json
 import os
@@ -73,6 +76,21 @@ def main():
 
     p = "results/H_forcing_convergence/forcing_conv.dat"
     data["forcing_transverse"] = {"file": p, "header": header(p), "rows": rows(p)}
+
+    sl = {}
+    for name in ("d2q9_cm", "d3q27_cm"):
+        p = f"results/B_shear_layer/shear_{name}.dat"
+        sl[name] = {"file": p, "header": header(p), "rows": rows(p)}
+    data["shear_layer"] = sl
+
+    cav = {}
+    for re_ in (100, 400, 1000):
+        for lat in ("d2q9", "d3q27"):
+            for tail in ("", "_full"):
+                p = f"results/C_cavity/cavity_re{re_}_{lat}_cm{tail}.dat"
+                if os.path.exists(os.path.join(REF, p)):
+                    cav[f"re{re_}_{lat}{tail}"] = {"file": p, "header": header(p), "rows": rows(p)}
+    data["cavity"] = cav
 
     with open(os.path.join(REF, "README.md")) as f:
         lines = f.read().splitlines()

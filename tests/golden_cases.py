@@ -304,3 +304,91 @@ def forcing_3d_sine(solver_cls, N, tau, umax=0.02, cap=4000000, dtype=abi.F64, t
     num = float(np.sum((ux - ana) ** 2 + uy ** 2 + uz ** 2))
     den = float(np.sum(ana ** 2))
     return math.sqrt(num / den), float(np.max(np.abs(ux))) / umax - 1.0
+
+
+def shear_layer_run(solver_cls, lattice, probes, dtype=abi.F64, **kw):
+    """validation/shear_layer.cpp:23-80 at its committed configuration: L = 256,
+    Re = 3e4, u0 = 0.02, CM shifted (Campaign.hpp OpCM), probes every
+    t0/40 = 320 steps.  Returns [(t/t0, E/E0, Psi/Psi0)] for probes 0..probes."""
+    L, Re, u0 = 256, 3e4, 0.02
+    nu = u0 * L / Re
+    t0 = int(L / u0)
+    probe = t0 // 40
+    cfg = abi.make_config(lattice, abi.OP_CM, abi.SHIFTED, dtype, (L, L, 1),
+                          omega=1.0 / (nu * 3.0 + 0.5))
+    s = solver_cls(cfg, **kw)
+    try:
+        s.initialize_field(*cases.shear_layer(cfg, u0))
+        e0, w0, _ = s.energy_enstrophy()
+        out = [(0.0, 1.0, 1.0)]
+        for k in range(1, probes + 1):
+            s.step(probe)
+            e, w, ok = s.energy_enstrophy()
+            assert ok
+            out.append((k * probe / t0, e / e0, w / w0))
+    finally:
+        s.close()
+    return out
+
+
+def cavity_run(solver_cls, lattice, N=129, Re=100.0, U=0.02, dtype=abi.F64, probe=500,
+               cap=4_000_000, rel=1e-11, restol=0.0, log=None, **kw):
+    """validation/cavity_cm.cpp:180-290: an N x N lid-driven cavity (periodic
+    z, nz = 1), regularised walls on the edge nodes (lid at (U, 0, 0), corners
+    at rest), CM shifted, nu = U (N - 1) / Re, started from rest at rho = 1,
+    stepped in blocks of `probe`.  Stops when the centre node's u_x changes by
+    less than rel * (|u| + 1e-12) between two probes (the default), or with
+    restol > 0 (its -restol option, :230-246) when the relative L2 change of
+    the whole velocity field over one probe falls below restol.  Returns
+    (u/U along x = N/2, v/U along y = N/2, steps taken)."""
+    nu = U * (N - 1) / Re
+    cfg = abi.make_config(lattice, abi.OP_CM, abi.SHIFTED, dtype, (N, N, 1),
+                          periodic=(False, False, True), omega=1.0 / (3.0 * nu + 0.5))
+    nrm, tag, table = cases.cavity_walls(N, U)
+    s = solver_cls(cfg, **kw)
+    try:
+        s.set_regularized_walls(nrm, tag, table)
+        s.initialize(1.0)
+        taken, prev = 0, 0.0
+        pu = pv = None
+        for t in range(0, cap, probe):
+            s.step(probe)
+            taken += probe
+            _, ux, uy, _ = s.compute_macroscopic()
+            ux = interior2d(cfg, ux)
+            uy = interior2d(cfg, uy)
+            cur = ux[N // 2, N // 2]
+            assert math.isfinite(cur), f"diverged at step {taken}"
+            if log and taken % 50_000 == 0:
+                log(f"cavity N={N} Re={Re:g}: step {taken}, centre u {cur:.12e}")
+            if restol > 0.0:
+                if pu is None:
+                    pu, pv = np.zeros_like(ux), np.zeros_like(uy)
+                num = float(np.sum((ux - pu) ** 2 + (uy - pv) ** 2))
+                den = float(np.sum(ux * ux + uy * uy))
+                res = math.sqrt(num / max(den, 1e-300))
+                pu, pv = ux, uy
+                if t > 0 and res < restol:
+                    break
+            elif t > 0 and abs(cur - prev) < rel * (abs(cur) + 1e-12):
+                break
+            prev = cur
+        return ux[:, N // 2] / U, uy[N // 2, :] / U, taken
+    finally:
+        s.close()
+
+
+def interior2d(cfg, a):
+    """(ny, nx) interior of a padded per-node array of a 2-D (nz = 1) box."""
+    nx, ny, nz, hx, hy, hz, sx, sy, sz, _ = abi.domain_of(cfg)
+    return np.asarray(a).reshape(sz, sy, sx)[hz, hy:hy + ny, hx:hx + nx]
+
+
+def lerp_at(v, s):
+    """validation/cavity_cm.cpp:161-166: linear interpolation of a profile
+    sampled at k / (len - 1)."""
+    p = s * (len(v) - 1)
+    i = int(p)
+    if i >= len(v) - 1:
+        return float(v[-1])
+    return float(v[i] + (p - i) * (v[i + 1] - v[i]))

@@ -57,3 +57,31 @@ def test_facade_per_node_force_policy():
     for N, tau, l2, _ in rows:
         want, _ = forcing_3d_sine(FluidSolver, int(N), float(tau))
         assert abs(float(l2) - want) <= 1e-6 * want, (N, tau, l2, want)
+
+
+def test_facade_regularized_walls():
+    """tests/cpp/cavity_facade.cpp: walls given as the reference's
+    cavity_cm.cpp gives them (a lambda -> WallSpec, velocities deduplicated by
+    the facade) step exactly as the same cavity set up through ctypes."""
+    import numpy as np
+    from golden_cases import interior2d
+    from m3lb_amd import abi, cases
+    from m3lb_amd.solver import FluidSolver
+    exe = os.path.join(os.path.dirname(EXE), "cavity_facade")
+    print("\n[gpu-case] facade cavity", flush=True)
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    N, Re, steps = 33, 100.0, 3000
+    r = subprocess.run([exe, str(N), str(Re), str(steps)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([float(v) for v in r.stdout.split()])
+    U = 0.02
+    nu = U * (N - 1) / Re
+    cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, abi.F64, (N, N, 1),
+                          periodic=(False, False, True), omega=1.0 / (3.0 * nu + 0.5))
+    with FluidSolver(cfg) as s:
+        s.set_regularized_walls(*cases.cavity_walls(N, U))
+        s.initialize(1.0)
+        s.step(steps)
+        ux = interior2d(cfg, s.compute_macroscopic()[1])
+    np.testing.assert_array_equal(got, ux[:, N // 2] / U)

@@ -181,3 +181,17 @@ def test_constant_force_field_is_uniform_guo():
             s.initialize_field(*cases.taylor_green_box(a, 0.03))
             s.step(7)
         assert np.array_equal(sa.gather_populations(), sb.gather_populations())
+
+
+@pytest.mark.parametrize("name,lattice", [("d2q9_cm", abi.D2Q9), ("d3q27_cm", abi.D3Q27)])
+def test_shear_layer_full_run(name, lattice):
+    """results/B_shear_layer/shear_{d2q9,d3q27}_cm.dat: all 41 probes to t0."""
+    from golden_cases import shear_layer_run
+    rows = GOLD["shear_layer"][name]["rows"]
+    got = shear_layer_run(FluidSolver, lattice, 40)
+    worst = 0.0
+    for (t, e, w), ref in zip(got, rows):
+        assert abs(t - ref[0]) < 1e-9
+        worst = max(worst, rel(e, ref[1]), rel(w, ref[2]))
+    print(f"shear layer {name}: worst relative difference {worst:.2e}")
+    assert worst < 1e-6

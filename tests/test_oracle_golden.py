@@ -203,3 +203,15 @@ def test_constant_force_field_is_uniform_guo():
         s.step(7)
     assert np.array_equal(fa.gather_populations(), fb.gather_populations())
     assert fa.total_momentum(0) == fb.total_momentum(0)
+
+
+def test_shear_layer_first_probes():
+    """results/B_shear_layer/shear_d2q9_cm.dat (shear_layer.cpp): energy and
+    enstrophy ratios of the first four probes (1280 steps) to the printed digits."""
+    from golden_cases import shear_layer_run
+    rows = GOLD["shear_layer"]["d2q9_cm"]["rows"]
+    got = shear_layer_run(OracleSolver, abi.D2Q9, 4, threads=8)
+    for (t, e, w), ref in zip(got, rows):
+        assert abs(t - ref[0]) < 1e-9
+        assert rel(e, ref[1]) < PRINT_TOL, (t, e, ref)
+        assert rel(w, ref[2]) < PRINT_TOL, (t, w, ref)

@@ -211,16 +211,23 @@ int m3lb_timestep(const m3lb_solver* s, uint64_t* t);
 
 /* collision().forcing = Guo{fx,fy,fz} and collision().omega* at run time. */
 int m3lb_set_force(m3lb_solver* s, double fx, double fy, double fz);
-/* set_regularized_walls(fn -> WallSpec) (FluidSolver.hpp:84-99, 372-460):
- * regularised velocity walls (Latt et al. 2008) on the nodes themselves.
- * normals: one code per INTERIOR node, x fastest -- 0 none, 1..6 the OUTWARD
- * normal +x, -x, +y, -y, +z, -z, 7 a corner (where two walls meet: density
+/* set_regularized_walls(fn -> WallSpec) (FluidSolver.hpp:84-103, 372-530):
+ * regularised walls (Latt et al. 2008) on the nodes themselves.
+ * normals: one M3LB_NRM_* code per INTERIOR node, x fastest -- 0 none, 1..6
+ * the OUTWARD normal +x, -x, +y, -y, +z, -z, 7 a corner (density
  * extrapolated along a wall, stress from finite-difference gradients unless
- * m3lb_set_fd_corners(s, 0)); tags: index into table (the reference's
- * deduplicated (ux, uy, uz, rho) table).  The nodes become M3LB_REGWALL.
- * Unknown directions are fixed from the flags as they are at this call;
- * m3lb_set_geometry afterwards removes the walls.  Straight walls and
- * corners only: the outflow codes and specular walls are not provided. */
+ * m3lb_set_fd_corners(s, 0)), 8 a back-pressure outlet on a +x face (rho from
+ * the table, normal velocity from the inverted closure, tangential velocity
+ * from upstream), 9 an outlet on any face (the donor neighbour's distribution
+ * rescaled to m3lb_set_outflow_density), 10 an equilibrium outlet (rho from
+ * the table, velocity from upstream); tags: index into table, the
+ * deduplicated (ux, uy, uz, rho) states.  The nodes become M3LB_REGWALL;
+ * unknown directions and outflow donors come from the flags as they are at
+ * this call (call after m3lb_set_geometry, which removes the walls).  A node
+ * that reads a neighbour (the outlets, a corner's stencil) sees an earlier
+ * node (lower padded index) after its update in the same step and a later
+ * one before it, the reference's serial order.  Codes 8 and 10 need two and
+ * one nodes upstream along -x.  A rejected call changes nothing. */
 typedef struct m3lb_wall {
   double ux, uy, uz, rho;
 } m3lb_wall;
@@ -228,6 +235,34 @@ int m3lb_set_regularized_walls(m3lb_solver* s, const uint8_t* normals, const uin
                                 const m3lb_wall* table, int32_t ntable);
 /* set_fd_corners(on) (FluidSolver.hpp:272); on by default. */
 int m3lb_set_fd_corners(m3lb_solver* s, int32_t on);
+/* set_force_correction(on) (:1311): the Guo odd part in the wall
+ * reconstruction; on by default. */
+int m3lb_set_force_correction(m3lb_solver* s, int32_t on);
+/* set_outflow_order(k) (:1324): tangential outlet velocity zero-gradient (1,
+ * the default) or linearly extrapolated (2). */
+int m3lb_set_outflow_order(m3lb_solver* s, int32_t order);
+/* set_outflow_density(rho) (:1330): the density code 9 imposes; 1 by default. */
+int m3lb_set_outflow_density(m3lb_solver* s, double rho);
+/* set_wall_velocity_scale(sc) (:303-312): every state's velocity is the table
+ * as given times sc (absolute, not compounding). */
+int m3lb_set_wall_velocity_scale(m3lb_solver* s, double scale);
+/* set_wall_density(rho) (:330-335): every state's density slot. */
+int m3lb_set_wall_density(m3lb_solver* s, double rho);
+/* set_wall_velocity_field(fn) (:353-367): one wall velocity per INTERIOR
+ * node, x fastest, read instead of the table (by the density closure, the
+ * reconstruction and the macroscopic readout; a corner's stencil keeps the
+ * table) until the solver is destroyed; its scale restarts at 1. */
+int m3lb_set_wall_velocity_field(m3lb_solver* s, const double* ux, const double* uy,
+                                 const double* uz);
+int m3lb_set_wall_velocity_field_scale(m3lb_solver* s, double scale);
+/* set_specular_walls(fn -> NormalCode) (:193-216): free-slip ghost cells;
+ * an axis normal 1..6 per INTERIOR node marks the cell M3LB_SPECWALL, 0 leaves
+ * it.  Call after m3lb_set_geometry.  Codes above 6 are rejected. */
+int m3lb_set_specular_walls(m3lb_solver* s, const uint8_t* normals);
+/* set_specular_nodes(fn -> SpecFace mask) (:236-270): on-node free-slip
+ * nodes (M3LB_SPECNODE), M3LB_SPEC_* bits per INTERIOR node; edges and
+ * corners carry several faces, never both faces of one axis. */
+int m3lb_set_specular_nodes(m3lb_solver* s, const uint8_t* faces);
 /* A forcing policy with a per-node F(n) (at(n, F), e.g. SinForce,
  * validation/forcing_3d.cpp:64-91): for handles created with
  * M3LB_FORCE_FIELD, three padded arrays of n_padded values (only Fluid nodes

@@ -158,11 +158,20 @@ template <class L, class F = NoForcing, class S = RawPopulations>
 using MRT = MomentCollision<L, F, S, false>;
 
 // ---- cells and domain (Flags.hpp:20-23, Domain.hpp:37-124) ------------------
-enum CellType : std::uint8_t { Fluid = 0, Solid = 1, Excluded = 2, RegWall = 3 };
+enum CellType : std::uint8_t {
+  Fluid = 0, Solid = 1, Excluded = 2, RegWall = 3, SpecWall = 4, SpecNode = 5
+};
 
-// wall normal codes, pointing out of the fluid (boundary/Regularized.hpp:98-103)
+// wall normal codes, pointing out of the fluid, and the outflow codes
+// (boundary/Regularized.hpp:97-142)
 enum WallNormal : std::uint8_t {
-  NrmNone = 0, NrmXp = 1, NrmXm = 2, NrmYp = 3, NrmYm = 4, NrmZp = 5, NrmZm = 6, NrmCorner = 7
+  NrmNone = 0, NrmXp = 1, NrmXm = 2, NrmYp = 3, NrmYm = 4, NrmZp = 5, NrmZm = 6, NrmCorner = 7,
+  NrmOutXp = 8, NrmOutFree = 9, NrmOutEq = 10
+};
+
+// on-node specular face mask bits (boundary/Specular.hpp:241-249)
+enum SpecFace : std::uint8_t {
+  SpecNone = 0, SpecXp = 1, SpecXm = 2, SpecYp = 4, SpecYm = 8, SpecZp = 16, SpecZm = 32
 };
 
 struct FlowState {  // src/solver/FluidSolver.hpp:30-32
@@ -265,6 +274,16 @@ class FluidSolver {
   FluidSolver(const FluidSolver&) = delete;
   FluidSolver& operator=(const FluidSolver&) = delete;
 
+  template <class Fn>
+  std::vector<std::uint8_t> interior_bytes(Fn fn) const {
+    std::vector<std::uint8_t> b(std::size_t(dom_.n_interior()));
+    std::size_t k = 0;
+    for (Index z = 0; z < dom_.nz; ++z)
+      for (Index y = 0; y < dom_.ny; ++y)
+        for (Index x = 0; x < dom_.nx; ++x) b[k++] = static_cast<std::uint8_t>(fn(x, y, z));
+    return b;
+  }
+
   // set_geometry(fn(x, y, z) -> CellType), FluidSolver.hpp:74-82
   template <class Fn>
   void set_geometry(Fn fn) {
@@ -276,12 +295,13 @@ class FluidSolver {
     check(m3lb_set_geometry(h_, fl.data()), "set_geometry");
   }
 
-  // set_regularized_walls(fn(x, y, z) -> WallSpec), FluidSolver.hpp:84-99,
-  // 372-460: velocity walls on the nodes themselves; the velocities are
+  // set_regularized_walls(fn(x, y, z) -> WallSpec), FluidSolver.hpp:84-103,
+  // 372-530: walls and outlets on the nodes themselves; the states are
   // deduplicated into a table.  Call after set_geometry.
   struct WallSpec {
     std::uint8_t normal = NrmNone;
     double ux = 0, uy = 0, uz = 0;
+    double rho = 1.0;  // read by the outflow codes only
   };
   template <class Fn>
   void set_regularized_walls(Fn fn) {
@@ -296,23 +316,60 @@ class FluidSolver {
           const WallSpec w = fn(x, y, z);
           if (w.normal == NrmNone) continue;
           std::size_t t = 0;
-          while (t < table.size() &&
-                 !(table[t].ux == w.ux && table[t].uy == w.uy && table[t].uz == w.uz))
+          while (t < table.size() && !(table[t].ux == w.ux && table[t].uy == w.uy &&
+                                       table[t].uz == w.uz && table[t].rho == w.rho))
             ++t;
           if (t == table.size()) {
-            if (t == 65536) throw std::runtime_error("set_regularized_walls: over 65536 velocities");
-            table.push_back(m3lb_wall{w.ux, w.uy, w.uz, 0.0});
+            if (t == 65536) throw std::runtime_error("set_regularized_walls: over 65536 wall states");
+            table.push_back(m3lb_wall{w.ux, w.uy, w.uz, w.rho});
           }
           nrm[k] = w.normal;
           tag[k] = std::uint16_t(t);
         }
-    if (table.empty()) table.push_back(m3lb_wall{0, 0, 0, 0});
+    if (table.empty()) table.push_back(m3lb_wall{0, 0, 0, 1});
     check(m3lb_set_regularized_walls(h_, nrm.data(), tag.data(), table.data(),
                                      std::int32_t(table.size())),
           "set_regularized_walls");
   }
   // set_fd_corners(on), FluidSolver.hpp:272
   void set_fd_corners(bool on) { check(m3lb_set_fd_corners(h_, on ? 1 : 0), "set_fd_corners"); }
+  // FluidSolver.hpp:303-367, 1311-1330
+  void set_force_correction(bool on) {
+    check(m3lb_set_force_correction(h_, on ? 1 : 0), "set_force_correction");
+  }
+  void set_outflow_order(int k) { check(m3lb_set_outflow_order(h_, k), "set_outflow_order"); }
+  void set_outflow_density(double r) { check(m3lb_set_outflow_density(h_, r), "set_outflow_density"); }
+  void set_wall_velocity_scale(double sc) {
+    check(m3lb_set_wall_velocity_scale(h_, sc), "set_wall_velocity_scale");
+  }
+  void set_wall_density(double r) { check(m3lb_set_wall_density(h_, r), "set_wall_density"); }
+  // fn(x, y, z) -> std::array<double, 3>, interior coordinates
+  template <class Fn>
+  void set_wall_velocity_field(Fn fn) {
+    const std::size_t ni = std::size_t(dom_.n_interior());
+    std::vector<double> u(ni), v(ni), w(ni);
+    std::size_t k = 0;
+    for (Index z = 0; z < dom_.nz; ++z)
+      for (Index y = 0; y < dom_.ny; ++y)
+        for (Index x = 0; x < dom_.nx; ++x, ++k) {
+          const auto c = fn(x, y, z);
+          u[k] = c[0]; v[k] = c[1]; w[k] = c[2];
+        }
+    check(m3lb_set_wall_velocity_field(h_, u.data(), v.data(), w.data()), "set_wall_velocity_field");
+  }
+  void set_wall_velocity_field_scale(double sc) {
+    check(m3lb_set_wall_velocity_field_scale(h_, sc), "set_wall_velocity_field_scale");
+  }
+  // set_specular_walls(fn(x, y, z) -> NrmXp..NrmZm or NrmNone), FluidSolver.hpp:193-216
+  template <class Fn>
+  void set_specular_walls(Fn fn) {
+    check(m3lb_set_specular_walls(h_, interior_bytes(fn).data()), "set_specular_walls");
+  }
+  // set_specular_nodes(fn(x, y, z) -> SpecFace mask), FluidSolver.hpp:236-270
+  template <class Fn>
+  void set_specular_nodes(Fn fn) {
+    check(m3lb_set_specular_nodes(h_, interior_bytes(fn).data()), "set_specular_nodes");
+  }
 
   void initialize(double rho0, double ux0 = 0, double uy0 = 0, double uz0 = 0) {
     sync_policy();

@@ -403,3 +403,6 @@ def box_walls(shape, U):
     nrm[count > 1] = NRM_CORNER
     table = np.array([[0.0, 0.0, 0.0, 0.0], [U, 0.0, 0.0, 0.0]])
     return nrm.reshape(-1), tag.reshape(-1), table
+
+NRM_OUT_XP, NRM_OUT_FREE, NRM_OUT_EQ = 8, 9, 10
+SPEC_XP, SPEC_XM, SPEC_YP, SPEC_YM, SPEC_ZP, SPEC_ZM = 1, 2, 4, 8, 16, 32

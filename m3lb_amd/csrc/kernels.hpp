@@ -22,10 +22,48 @@ struct Dom {
   int np;          // n_padded
 };
 
-// Fluid, or a regularised-wall node, which collides like one
-// (FluidSolver.hpp:557, 1054, 1215).
+// A node that holds a fluid state: Fluid, a regularised-wall node and an
+// on-node specular node (FluidSolver.hpp:557, 575, 1054, 1215).
 __host__ __device__ __forceinline__ bool fluid_like(uint8_t fl) {
-  return fl == M3LB_FLUID || fl == M3LB_REGWALL;
+  return fl == M3LB_FLUID || fl == M3LB_REGWALL || fl == M3LB_SPECNODE;
+}
+
+// ---------------------------------------------------------------------------
+// Specular reflection (src/boundary/Specular.hpp).  The mirror of direction i
+// about a set of axes is the direction with those components of c_i negated
+// (MirrorTable<L>, :104-121); every index is a compile-time constant.
+// ---------------------------------------------------------------------------
+template <int Q, int i, int fx, int fy, int fz>
+__host__ __device__ constexpr int mirror_of() {
+  return dir_of<Q>((fx ? -cvel<Q>(i, 0) : cvel<Q>(i, 0)) + 1,
+                   (fy ? -cvel<Q>(i, 1) : cvel<Q>(i, 1)) + 1,
+                   (fz ? -cvel<Q>(i, 2) : cvel<Q>(i, 2)) + 1);
+}
+
+// face_sign (Specular.hpp:253-258): outward sign of a face mask on axis a
+__device__ __forceinline__ int face_sign(uint8_t faces, int a) {
+  return (faces & (1u << (2 * a))) ? 1 : ((faces & (1u << (2 * a + 1))) ? -1 : 0);
+}
+
+// mirror_unknowns_faces (Specular.hpp:274-289): each direction that points
+// into the domain on a masked axis takes the value of its image through the
+// composition of those axis mirrors.
+template <class K>
+__device__ __forceinline__ void mirror_unknowns_faces(typename K::R* f, uint8_t faces) {
+  using R = typename K::R;
+  constexpr int Q = K::Q;
+  const int sx = face_sign(faces, 0), sy = face_sign(faces, 1), sz = face_sign(faces, 2);
+  R h[Q];
+  sfor<Q>([&](auto I) { h[int(I)] = f[int(I)]; });
+  sfor<Q>([&](auto I) {
+    constexpr int i = I;
+    constexpr int cx = cvel<Q>(i, 0), cy = cvel<Q>(i, 1), cz = cvel<Q>(i, 2);
+    const bool fx = cx * sx < 0, fy = cy * sy < 0, fz = cz * sz < 0;
+    f[i] = fx ? (fy ? (fz ? h[mirror_of<Q, i, 1, 1, 1>()] : h[mirror_of<Q, i, 1, 1, 0>()])
+                    : (fz ? h[mirror_of<Q, i, 1, 0, 1>()] : h[mirror_of<Q, i, 1, 0, 0>()]))
+              : (fy ? (fz ? h[mirror_of<Q, i, 0, 1, 1>()] : h[mirror_of<Q, i, 0, 1, 0>()])
+                    : (fz ? h[mirror_of<Q, i, 0, 0, 1>()] : h[i]));
+  });
 }
 
 struct StepArgs {
@@ -550,6 +588,12 @@ struct ReadArgs {
   const uint16_t* bc_tag;
   const void* bc_rho;
   const double* wtab;
+  const unsigned* bc_don;   // outflow donors (NrmOutFree)
+  const uint8_t* bc_onrm;   // their axis face codes
+  const void* wall_un;      // per-node wall velocity planes (compute type) or null
+  double un_scale, outflow_rho;
+  int out_order;
+  const uint8_t* spec_faces;  // on-node specular face masks, or null
 };
 
 __device__ __forceinline__ int shift_axis(int v, int h, int n, int s, int per) {
@@ -654,6 +698,28 @@ __global__ void __launch_bounds__(kBlock) k_macro(ReadArgs a) {
     r = double(density<K>(m)); ux = double(m.ux); uy = double(m.uy); uz = double(m.uz);
   } else if (a.flags[n] == M3LB_REGWALL && a.bc_nrm) {
     wall_macro<K>(a, unsigned(n), r, ux, uy, uz);
+  } else if (a.flags[n] == M3LB_SPECNODE && a.spec_faces) {
+    // macro_kernel's SpecNode path (FluidSolver.hpp:1057-1077): raw moments
+    // of the mirrored incoming set
+    int x, y, z;
+    interior_coords<K, 0>(d, n, x, y, z);
+    const Offs o = offsets(d, x, y, z);
+    R f[K::Q];
+    const T* A = static_cast<const T*>(a.f);
+    if (a.parity == 0) load_node<K, 0>(A, a.stride, unsigned(n), o, a.inv_scale, f);
+    else load_node<K, 1>(A, a.stride, unsigned(n), o, a.inv_scale, f);
+    mirror_unknowns_faces<K>(f, a.spec_faces[n]);
+    if constexpr (K::SHIFTED) sfor<K::Q>([&](auto I) { f[int(I)] += weight<K>(int(I)); });
+    R sm = R(0), mx = R(0), my = R(0), mz = R(0);
+    sfor<K::Q>([&](auto I) {
+      constexpr int i = I;
+      sm += f[i];
+      mx += f[i] * R(cvel<K::Q>(i, 0));
+      my += f[i] * R(cvel<K::Q>(i, 1));
+      mz += f[i] * R(cvel<K::Q>(i, 2));
+    });
+    const R ir = R(1) / sm;
+    r = double(sm); ux = double(mx * ir); uy = double(my * ir); uz = double(mz * ir);
   }
   if (a.out2[0]) a.out2[0][n] = r;
   if (a.out2[1]) a.out2[1][n] = ux;
@@ -712,6 +778,8 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mass(ReadArgs a) {
     const T* A = static_cast<const T*>(a.f);
     if (a.parity == 0) load_node<K, 0>(A, a.stride, unsigned(n), o, a.inv_scale, f);
     else load_node<K, 1>(A, a.stride, unsigned(n), o, a.inv_scale, f);
+    // the streamed unknowns of a SpecNode hold nothing until mirrored (:1218-1220)
+    if (a.flags[n] == M3LB_SPECNODE && a.spec_faces) mirror_unknowns_faces<K>(f, a.spec_faces[n]);
     const Macro<R> m = macroscopic<K>(f, p);
     const R r = density<K>(m);
     acc[0] += double(r);
@@ -741,13 +809,17 @@ __global__ void __launch_bounds__(kBlock) k_reduce_mass(ReadArgs a) {
 // neighbour's incoming set from before the step.
 // ---------------------------------------------------------------------------
 constexpr int kNrmCorner = 7;
+constexpr int kNrmOutXp = 8;
+constexpr int kNrmOutFree = 9;
+constexpr int kNrmOutEq = 10;
+constexpr int kWallReads = 6;  // neighbour sets a wall node may read (3 axes x 2)
 
 struct WallArgs {
   void* f;
   uint64_t stride;
   const uint8_t* flags;
   Dom d;
-  const unsigned* list;  // padded index of every wall node, ascending
+  const unsigned* list;  // padded index of each wall node of this launch
   int nwall;
   int fd_corners;
   const uint8_t* bc_nrm;
@@ -756,14 +828,24 @@ struct WallArgs {
   const uint8_t* bc_ext;   // corners: extrapolation axis 1..6, 0 = none
   void* bc_rho;            // compute type
   const double* wtab;      // (ux, uy, uz, rho) per table entry
+  const unsigned* bc_don;  // NrmOutFree donors
+  const void* wall_un;     // per-node wall velocity planes (compute type) or null
+  double un_scale, outflow_rho;
+  int out_order, force_bc;
+  // Where each of a wall node's neighbour reads comes from: -1 = the live
+  // slots now, k >= 0 = row k of snap (that neighbour's incoming set taken
+  // before the step); kWallReads entries per list entry.
+  const int* rd;
+  const void* snap;
   double omega, omega_m, omega_b, F[3];
   const void* ff;
+  void* macro[4];  // step(store_macroscopic): rho, ux, uy, uz or null
   float scale[27], inv_scale[27];
 };
 
 __host__ __device__ __forceinline__ void normal_of(int code, int n[3]) {
   n[0] = n[1] = n[2] = 0;
-  if (code == 1) n[0] = 1;
+  if (code == 1 || code == kNrmOutXp) n[0] = 1;
   else if (code == 2) n[0] = -1;
   else if (code == 3) n[1] = 1;
   else if (code == 4) n[1] = -1;
@@ -784,20 +866,37 @@ __device__ __forceinline__ typename K::R wall_eq(typename K::R rho, typename K::
   return weight<K>(i) * rho * (R(1) + phi);
 }
 
-// Regularized::density
+// Regularized::density and, with rho given, normal_velocity (the same
+// partition of the set inverted)
 template <class K>
-__device__ __forceinline__ typename K::R bc_density(const typename K::R* f, const int nrm[3],
-                                                    const typename K::R u[3]) {
+__device__ __forceinline__ void bc_partition(const typename K::R* f, const int nrm[3],
+                                             typename K::R& out, typename K::R& tang) {
   using R = typename K::R;
-  R out = R(0), tang = R(0);
+  out = R(0);
+  tang = R(0);
   sfor<K::Q>([&](auto I) {
     constexpr int i = I;
     const int cn = cvel<K::Q>(i, 0) * nrm[0] + cvel<K::Q>(i, 1) * nrm[1] + cvel<K::Q>(i, 2) * nrm[2];
     if (cn > 0) out += f[i];
     else if (cn == 0) tang += f[i];
   });
+}
+template <class K>
+__device__ __forceinline__ typename K::R bc_density(const typename K::R* f, const int nrm[3],
+                                                    const typename K::R u[3]) {
+  using R = typename K::R;
+  R out, tang;
+  bc_partition<K>(f, nrm, out, tang);
   const R uperp = u[0] * R(nrm[0]) + u[1] * R(nrm[1]) + u[2] * R(nrm[2]);
   return (R(2) * out + tang) / (R(1) + uperp);
+}
+template <class K>
+__device__ __forceinline__ typename K::R bc_normal_velocity(const typename K::R* f,
+                                                            const int nrm[3], typename K::R rho) {
+  using R = typename K::R;
+  R out, tang;
+  bc_partition<K>(f, nrm, out, tang);
+  return (R(2) * out + tang) / rho - R(1);
 }
 
 template <class K, int i>
@@ -858,12 +957,33 @@ __device__ __forceinline__ void bc_apply_with_stress(typename K::R* f, typename 
   });
 }
 
+// The table velocity of wall node n (the finite-difference stencil's wall
+// neighbours always use it, FluidSolver.hpp:961-966)
 template <class K>
-__device__ __forceinline__ void wall_velocity(const WallArgs& a, unsigned n, typename K::R u[3]) {
-  const double* w = a.wtab + 4 * size_t(a.bc_tag[n]);
+__device__ __forceinline__ void table_velocity(const double* wtab, const uint16_t* tag,
+                                               unsigned n, typename K::R u[3]) {
+  const double* w = wtab + 4 * size_t(tag[n]);
   u[0] = typename K::R(w[0]);
   u[1] = typename K::R(w[1]);
   u[2] = typename K::R(w[2]);
+}
+
+// The wall velocity of node n: the per-node field when one is set
+// (set_wall_velocity_field), else its table entry (FluidSolver.hpp:686-689).
+template <class K>
+__device__ __forceinline__ void wall_velocity(const double* wtab, const uint16_t* tag,
+                                              const void* un, double un_scale, int np,
+                                              unsigned n, typename K::R u[3]) {
+  using R = typename K::R;
+  if (un) {
+    const R* w = static_cast<const R*>(un);
+    const R s = R(un_scale);
+    u[0] = s * w[n];
+    u[1] = s * w[size_t(np) + n];
+    u[2] = s * w[2 * size_t(np) + n];
+    return;
+  }
+  table_velocity<K>(wtab, tag, n, u);
 }
 
 // The incoming set of padded node n at parity P, unshifted to f.
@@ -877,7 +997,31 @@ __device__ __forceinline__ void load_raw(const void* f, uint64_t stride, const D
   if constexpr (K::SHIFTED) sfor<K::Q>([&](auto I) { out[int(I)] += weight<K>(int(I)); });
 }
 
-// corner_density<P>, first half: rho of every straight-wall node.
+// Sum and mean velocity of an (unshifted) set; guard: 0 instead of dividing
+// by a non-positive sum
+template <class K>
+__device__ __forceinline__ typename K::R set_velocity(const typename K::R* g, typename K::R v[3],
+                                                      bool guard) {
+  using R = typename K::R;
+  R sm = R(0), mx = R(0), my = R(0), mz = R(0);
+  sfor<K::Q>([&](auto I) {
+    constexpr int i = I;
+    sm += g[i];
+    mx += g[i] * R(cvel<K::Q>(i, 0));
+    my += g[i] * R(cvel<K::Q>(i, 1));
+    mz += g[i] * R(cvel<K::Q>(i, 2));
+  });
+  const R ir = guard ? ((sm > R(0)) ? R(1) / sm : R(0)) : R(1) / sm;
+  v[0] = mx * ir; v[1] = my * ir; v[2] = mz * ir;
+  return sm;
+}
+
+__device__ __forceinline__ unsigned interior_id(const Dom& d, int x, int y, int z) {
+  return unsigned(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx);
+}
+
+// corner_density<P>, first half: rho of every straight-wall node (imposed at
+// the outflow codes).
 template <class K, int P>
 __global__ void __launch_bounds__(kBlock) k_wall_density(WallArgs a) {
   using R = typename K::R;
@@ -886,13 +1030,22 @@ __global__ void __launch_bounds__(kBlock) k_wall_density(WallArgs a) {
   const unsigned n = a.list[k];
   const int code = a.bc_nrm[n];
   if (code == kNrmCorner || code == 0) return;
+  R* rho = static_cast<R*>(a.bc_rho);
+  if (code == kNrmOutXp || code == kNrmOutEq) {
+    rho[n] = R(a.wtab[4 * size_t(a.bc_tag[n]) + 3]);
+    return;
+  }
+  if (code == kNrmOutFree) {
+    rho[n] = R(a.outflow_rho);
+    return;
+  }
   R f[K::Q];
   load_raw<K, P>(a.f, a.stride, a.d, n, a.inv_scale, f);
   int nrm[3];
   normal_of(code, nrm);
   R uw[3];
-  wall_velocity<K>(a, n, uw);
-  static_cast<R*>(a.bc_rho)[n] = bc_density<K>(f, nrm, uw);
+  wall_velocity<K>(a.wtab, a.bc_tag, a.wall_un, a.un_scale, a.d.np, n, uw);
+  rho[n] = bc_density<K>(f, nrm, uw);
 }
 
 // corner_density<P>, second half: each corner 2 rho(n1) - rho(n2) along its
@@ -916,15 +1069,37 @@ __global__ void __launch_bounds__(kBlock) k_corner_density(WallArgs a) {
   const int ax = (e - 1) >> 1, sg = (e & 1) ? 1 : -1;  // 1:+x 2:-x 3:+y 4:-y 5:+z 6:-z
   int v[3] = {0, 0, 0};
   v[ax] = sg;
-  auto id = [&](int xx, int yy, int zz) {
-    return unsigned(((zz + d.hz) * d.sy + (yy + d.hy)) * d.sx + xx + d.hx);
-  };
-  const unsigned n1 = id(x + v[0], y + v[1], z + v[2]);
-  const unsigned n2 = id(x + 2 * v[0], y + 2 * v[1], z + 2 * v[2]);
+  const unsigned n1 = interior_id(d, x + v[0], y + v[1], z + v[2]);
+  const unsigned n2 = interior_id(d, x + 2 * v[0], y + 2 * v[1], z + 2 * v[2]);
   rho[n] = R(2) * rho[n1] - rho[n2];
 }
 
-// The wall path of run_step for one wall node: rebuild, collide, store.
+// The neighbour sets the wall nodes read, taken before the step (see rd).
+struct SnapArgs {
+  const void* f;
+  uint64_t stride;
+  Dom d;
+  const unsigned* nodes;
+  int count;
+  void* out;  // count x Q, compute type, unshifted
+  float inv_scale[27];
+};
+
+template <class K, int P>
+__global__ void __launch_bounds__(kBlock) k_snapshot(SnapArgs a) {
+  using R = typename K::R;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.count) return;
+  R g[K::Q];
+  load_raw<K, P>(a.f, a.stride, a.d, a.nodes[k], a.inv_scale, g);
+  R* o = static_cast<R*>(a.out) + size_t(k) * K::Q;
+  sfor<K::Q>([&](auto I) { o[int(I)] = g[int(I)]; });
+}
+
+// The wall path of run_step for one wall node (FluidSolver.hpp:793-1033):
+// velocity walls, corners and the outflow codes; rebuild, collide, store
+// (the population-extrapolation and equilibrium outlets store without
+// colliding).
 template <class K, int P>
 __global__ void __launch_bounds__(kBlock) k_walls(WallArgs a) {
   using R = typename K::R;
@@ -944,94 +1119,255 @@ __global__ void __launch_bounds__(kBlock) k_walls(WallArgs a) {
   const int code = a.bc_nrm[n];
   int nrm[3];
   normal_of(code, nrm);
+  const size_t tg = a.bc_tag[n];
   R uw[3];
-  wall_velocity<K>(a, n, uw);
+  wall_velocity<K>(a.wtab, a.bc_tag, a.wall_un, a.un_scale, d.np, n, uw);
   Params<R> p;
   p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
   p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
   force_at<K>(p, a.ff, a.stride, n);
   R Fv[3] = {R(0), R(0), R(0)};
-  if constexpr (K::FORCE) { Fv[0] = p.F[0]; Fv[1] = p.F[1]; Fv[2] = p.F[2]; }
-  const R rw = (code == kNrmCorner) ? static_cast<const R*>(a.bc_rho)[n] : bc_density<K>(f, nrm, uw);
-  if (a.fd_corners && code == kNrmCorner) {
-    const int ic[3] = {x, y, z}, nn[3] = {d.nx, d.ny, d.nz}, per[3] = {d.px, d.py, d.pz};
-    R grad[3][3] = {{R(0), R(0), R(0)}, {R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
-    auto vel_at = [&](int cx, int cy, int cz, R out[3]) {
-      const unsigned m = unsigned(((cz + d.hz) * d.sy + (cy + d.hy)) * d.sx + cx + d.hx);
-      if (a.flags[m] == M3LB_REGWALL) {
-        wall_velocity<K>(a, m, out);
-        return;
-      }
-      R g[Q];
+  if constexpr (K::FORCE) {
+    if (a.force_bc) { Fv[0] = p.F[0]; Fv[1] = p.F[1]; Fv[2] = p.F[2]; }
+  }
+  // neighbour read r of this node: the snapshot row or the live slots
+  auto incoming = [&](int r, unsigned m, R g[Q]) {
+    const int src = a.rd ? a.rd[size_t(k) * kWallReads + r] : -1;
+    if (src >= 0) {
+      const R* sp = static_cast<const R*>(a.snap) + size_t(src) * Q;
+      sfor<Q>([&](auto I) { g[int(I)] = sp[int(I)]; });
+    } else {
       load_raw<K, P>(a.f, a.stride, d, m, a.inv_scale, g);
-      R sm = R(0), mx = R(0), my = R(0), mz = R(0);
-      sfor<Q>([&](auto I) {
-        constexpr int i = I;
-        sm += g[i];
-        mx += g[i] * R(cvel<Q>(i, 0));
-        my += g[i] * R(cvel<Q>(i, 1));
-        mz += g[i] * R(cvel<Q>(i, 2));
-      });
-      const R ir = R(1) / sm;
-      out[0] = mx * ir; out[1] = my * ir; out[2] = mz * ir;
-    };
-    for (int ax = 0; ax < K::D; ++ax) {
-      if (nn[ax] < 3) continue;
-      int dir = 0;
-      if (!per[ax]) {
-        if (ic[ax] == 0) dir = 1;
-        else if (ic[ax] == nn[ax] - 1) dir = -1;
-      }
-      int c1[3] = {ic[0], ic[1], ic[2]}, c2[3] = {ic[0], ic[1], ic[2]};
-      R u1[3], u2[3];
-      if (dir != 0) {
-        c1[ax] += dir; c2[ax] += 2 * dir;
-        vel_at(c1[0], c1[1], c1[2], u1);
-        vel_at(c2[0], c2[1], c2[2], u2);
-        for (int b = 0; b < K::D; ++b)
-          grad[ax][b] = R(dir) * (R(-1.5) * uw[b] + R(2) * u1[b] - R(0.5) * u2[b]);
-      } else {
-        c1[ax] = (ic[ax] + 1) % nn[ax];
-        c2[ax] = (ic[ax] - 1 + nn[ax]) % nn[ax];
-        vel_at(c1[0], c1[1], c1[2], u1);
-        vel_at(c2[0], c2[1], c2[2], u2);
-        for (int b = 0; b < K::D; ++b) grad[ax][b] = R(0.5) * (u1[b] - u2[b]);
-      }
     }
-    // Regularized::stress_from_gradient with the shear rate
-    const R kf = -R(2) * (R(1) / R(3)) * rw / p.omega;
-    const R Pi[6] = {kf * grad[0][0], kf * grad[1][1], kf * grad[2][2],
-                     kf * R(0.5) * (grad[0][1] + grad[1][0]), kf * R(0.5) * (grad[0][2] + grad[2][0]),
-                     kf * R(0.5) * (grad[1][2] + grad[2][1])};
-    bc_apply_with_stress<K>(f, rw, uw, Pi);
+  };
+  R ur[3] = {uw[0], uw[1], uw[2]};
+  R rw = R(0);
+  bool collide_it = true;
+  if (code == kNrmOutFree) {
+    // the donor's distribution rescaled to the imposed density (:843-880)
+    R g[Q];
+    incoming(0, a.bc_don[n], g);
+    R sm = R(0);
+    sfor<Q>([&](auto I) { sm += g[int(I)]; });
+    if (sm > R(0)) {
+      const R sc = R(a.outflow_rho) / sm;
+      sfor<Q>([&](auto I) { f[int(I)] = g[int(I)] * sc; });
+    }
+    collide_it = false;
+  } else if (code == kNrmOutEq) {
+    // equilibrium at the imposed rho and the upstream velocity (:881-911)
+    const R rw2 = R(a.wtab[4 * tg + 3]);
+    R g[Q], mu[3];
+    incoming(0, interior_id(d, x - 1, y, z), g);
+    set_velocity<K>(g, mu, true);
+    sfor<Q>([&](auto I) { f[int(I)] = wall_eq<K, int(I)>(rw2, mu[0], mu[1], mu[2]); });
+    collide_it = false;
+  } else if (code == kNrmOutXp) {
+    // back-pressure outlet (:912-951): rho imposed, the normal velocity from
+    // the inverted closure, the tangential ones from upstream
+    rw = R(a.wtab[4 * tg + 3]);
+    R g[Q], ut[3];
+    incoming(0, interior_id(d, x - 1, y, z), g);
+    set_velocity<K>(g, ut, false);
+    if (a.out_order >= 2) {
+      R v2[3];
+      incoming(1, interior_id(d, x - 2, y, z), g);
+      set_velocity<K>(g, v2, false);
+      for (int c = 0; c < 3; ++c) ut[c] = R(2) * ut[c] - v2[c];
+    }
+    const R un = bc_normal_velocity<K>(f, nrm, rw);
+    R utn = R(0);
+    for (int c = 0; c < 3; ++c) utn += ut[c] * R(nrm[c]);
+    for (int c = 0; c < 3; ++c) ur[c] = ut[c] + (un - utn) * R(nrm[c]);
   } else {
-    bc_apply<K>(f, rw, uw, a.bc_unk[n], Fv);
+    rw = (code == kNrmCorner) ? static_cast<const R*>(a.bc_rho)[n] : bc_density<K>(f, nrm, ur);
+  }
+  if (collide_it) {
+    if (a.fd_corners && code == kNrmCorner) {
+      const int ic[3] = {x, y, z}, nn[3] = {d.nx, d.ny, d.nz}, per[3] = {d.px, d.py, d.pz};
+      R grad[3][3] = {{R(0), R(0), R(0)}, {R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
+      // velocity at a stencil node: imposed (table) at a wall node, else from
+      // its incoming set
+      auto vel_at = [&](int r, int cx, int cy, int cz, R out[3]) {
+        const unsigned m = interior_id(d, cx, cy, cz);
+        if (a.flags[m] == M3LB_REGWALL) {
+          table_velocity<K>(a.wtab, a.bc_tag, m, out);
+          return;
+        }
+        R g[Q];
+        incoming(r, m, g);
+        set_velocity<K>(g, out, false);
+      };
+      for (int ax = 0; ax < K::D; ++ax) {
+        if (nn[ax] < 3) continue;
+        int dir = 0;
+        if (!per[ax]) {
+          if (ic[ax] == 0) dir = 1;
+          else if (ic[ax] == nn[ax] - 1) dir = -1;
+        }
+        int c1[3] = {ic[0], ic[1], ic[2]}, c2[3] = {ic[0], ic[1], ic[2]};
+        R u1[3], u2[3];
+        if (dir != 0) {
+          c1[ax] += dir; c2[ax] += 2 * dir;
+          vel_at(2 * ax, c1[0], c1[1], c1[2], u1);
+          vel_at(2 * ax + 1, c2[0], c2[1], c2[2], u2);
+          for (int b = 0; b < K::D; ++b)
+            grad[ax][b] = R(dir) * (R(-1.5) * uw[b] + R(2) * u1[b] - R(0.5) * u2[b]);
+        } else {
+          c1[ax] = (ic[ax] + 1) % nn[ax];
+          c2[ax] = (ic[ax] - 1 + nn[ax]) % nn[ax];
+          vel_at(2 * ax, c1[0], c1[1], c1[2], u1);
+          vel_at(2 * ax + 1, c2[0], c2[1], c2[2], u2);
+          for (int b = 0; b < K::D; ++b) grad[ax][b] = R(0.5) * (u1[b] - u2[b]);
+        }
+      }
+      // Regularized::stress_from_gradient with the shear rate
+      const R kf = -R(2) * (R(1) / R(3)) * rw / p.omega;
+      const R Pi[6] = {kf * grad[0][0], kf * grad[1][1], kf * grad[2][2],
+                       kf * R(0.5) * (grad[0][1] + grad[1][0]), kf * R(0.5) * (grad[0][2] + grad[2][0]),
+                       kf * R(0.5) * (grad[1][2] + grad[2][1])};
+      bc_apply_with_stress<K>(f, rw, ur, Pi);
+    } else {
+      bc_apply<K>(f, rw, ur, a.bc_unk[n], Fv);
+    }
   }
   if constexpr (K::SHIFTED) sfor<Q>([&](auto I) { f[int(I)] -= weight<K>(int(I)); });
-  collide_fused<K>(f, p);
+  if (collide_it) {
+    const Macro<R> m = collide_fused<K>(f, p);
+    if (a.macro[0]) {
+      static_cast<R*>(a.macro[0])[n] = density<K>(m);
+      static_cast<R*>(a.macro[1])[n] = m.ux;
+      static_cast<R*>(a.macro[2])[n] = m.uy;
+      static_cast<R*>(a.macro[3])[n] = m.uz;
+    }
+  }
   store_node<K, P>(A, a.stride, n, o, a.scale, f);
 }
 
-// macro_kernel's RegWall path (FluidSolver.hpp:1076-1185): the imposed
-// velocity and the closure density; a corner reports the last pass's rho.
+// Specular cells over a compact list (FluidSolver.hpp:745-756, 782-793).  A
+// ghost (SpecWall) permutes its incoming set about its normal's axis and
+// stores it back without colliding; an on-node node (SpecNode) mirrors its
+// unknown directions and collides as fluid.  Both touch only their own slots.
+struct SpecArgs {
+  void* f;
+  uint64_t stride;
+  const uint8_t* flags;
+  Dom d;
+  const unsigned* list;
+  int count;
+  const uint8_t* spec_nrm;
+  const uint8_t* spec_faces;
+  double omega, omega_m, omega_b, F[3];
+  const void* ff;
+  void* macro[4];
+  float scale[27], inv_scale[27];
+};
+
+template <class K, int P>
+__global__ void __launch_bounds__(kBlock) k_specular(SpecArgs a) {
+  using R = typename K::R;
+  using T = typename K::T;
+  constexpr int Q = K::Q;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.count) return;
+  const unsigned n = a.list[k];
+  const Dom& d = a.d;
+  int x, y, z;
+  interior_coords<K, 0>(d, int(n), x, y, z);
+  const Offs o = offsets(d, x, y, z);
+  T* A = static_cast<T*>(a.f);
+  R f[Q];
+  load_node<K, P>(A, a.stride, n, o, a.inv_scale, f);
+  if (a.flags[n] == M3LB_SPECWALL) {
+    // the weights are invariant under an axis mirror, so the permutation
+    // commutes with the shift
+    int nv[3];
+    normal_of(a.spec_nrm[n], nv);
+    const int ax = nv[0] ? 0 : (nv[1] ? 1 : 2);
+    R g[Q];
+    sfor<Q>([&](auto I) {
+      constexpr int i = I;
+      g[i] = ax == 0 ? f[mirror_of<Q, i, 1, 0, 0>()]
+                     : (ax == 1 ? f[mirror_of<Q, i, 0, 1, 0>()] : f[mirror_of<Q, i, 0, 0, 1>()]);
+    });
+    store_node<K, P>(A, a.stride, n, o, a.scale, g);
+    return;
+  }
+  mirror_unknowns_faces<K>(f, a.spec_faces[n]);
+  Params<R> p;
+  p.omega = R(a.omega); p.omega_m = R(a.omega_m); p.omega_b = R(a.omega_b);
+  p.F[0] = R(a.F[0]); p.F[1] = R(a.F[1]); p.F[2] = R(a.F[2]);
+  force_at<K>(p, a.ff, a.stride, n);
+  const Macro<R> m = collide_fused<K>(f, p);
+  store_node<K, P>(A, a.stride, n, o, a.scale, f);
+  if (a.macro[0]) {
+    static_cast<R*>(a.macro[0])[n] = density<K>(m);
+    static_cast<R*>(a.macro[1])[n] = m.ux;
+    static_cast<R*>(a.macro[2])[n] = m.uy;
+    static_cast<R*>(a.macro[3])[n] = m.uz;
+  }
+}
+
+// macro_kernel's RegWall path (FluidSolver.hpp:1078-1185): the imposed (or
+// outflow) velocity and the closure (or imposed) density; a corner reports
+// the last pass's rho.
 template <class K>
 __device__ void wall_macro(const ReadArgs& a, unsigned n, double& r, double& ux, double& uy,
                            double& uz) {
   using R = typename K::R;
+  constexpr int Q = K::Q;
+  const Dom& d = a.d;
   const int code = a.bc_nrm[n];
-  const double* w = a.wtab + 4 * size_t(a.bc_tag[n]);
-  const R uw[3] = {R(w[0]), R(w[1]), R(w[2])};
-  ux = double(uw[0]); uy = double(uw[1]); uz = double(uw[2]);
-  if (code == kNrmCorner) {
-    r = double(static_cast<const R*>(a.bc_rho)[n]);
-    return;
-  }
-  R f[K::Q];
-  if (a.parity == 0) load_raw<K, 0>(a.f, a.stride, a.d, n, a.inv_scale, f);
-  else load_raw<K, 1>(a.f, a.stride, a.d, n, a.inv_scale, f);
+  const size_t tg = a.bc_tag[n];
+  R uw[3];
+  wall_velocity<K>(a.wtab, a.bc_tag, a.wall_un, a.un_scale, d.np, n, uw);
+  R fw[Q];
+  auto raw = [&](unsigned m, R* g) {
+    if (a.parity == 0) load_raw<K, 0>(a.f, a.stride, d, m, a.inv_scale, g);
+    else load_raw<K, 1>(a.f, a.stride, d, m, a.inv_scale, g);
+  };
+  raw(n, fw);
   int nrm[3];
   normal_of(code, nrm);
-  r = double(bc_density<K>(f, nrm, uw));
+  int x, y, z;
+  interior_coords<K, 0>(d, int(n), x, y, z);
+  R rw;
+  if (code == kNrmOutFree) {
+    rw = R(a.outflow_rho);
+    int onr[3];
+    normal_of(a.bc_onrm[n], onr);
+    R g[Q], v[3];
+    raw(a.bc_don[n], g);
+    set_velocity<K>(g, v, true);
+    const R un = bc_normal_velocity<K>(fw, onr, rw);
+    R utn = R(0);
+    for (int c = 0; c < 3; ++c) utn += v[c] * R(onr[c]);
+    for (int c = 0; c < 3; ++c) uw[c] = v[c] + (un - utn) * R(onr[c]);
+  } else if (code == kNrmOutEq) {
+    rw = R(a.wtab[4 * tg + 3]);
+    R g[Q];
+    raw(interior_id(d, x - 1, y, z), g);
+    set_velocity<K>(g, uw, true);
+  } else if (code == kNrmOutXp) {
+    rw = R(a.wtab[4 * tg + 3]);
+    R g[Q], ut[3];
+    raw(interior_id(d, x - 1, y, z), g);
+    set_velocity<K>(g, ut, false);
+    if (a.out_order >= 2) {
+      R v2[3];
+      raw(interior_id(d, x - 2, y, z), g);
+      set_velocity<K>(g, v2, false);
+      for (int c = 0; c < 3; ++c) ut[c] = R(2) * ut[c] - v2[c];
+    }
+    const R un = bc_normal_velocity<K>(fw, nrm, rw);
+    R utn = R(0);
+    for (int c = 0; c < 3; ++c) utn += ut[c] * R(nrm[c]);
+    for (int c = 0; c < 3; ++c) uw[c] = ut[c] + (un - utn) * R(nrm[c]);
+  } else {
+    rw = (code == kNrmCorner) ? static_cast<const R*>(a.bc_rho)[n] : bc_density<K>(fw, nrm, uw);
+  }
+  r = double(rw);
+  ux = double(uw[0]); uy = double(uw[1]); uz = double(uw[2]);
 }
 
 template <class K>

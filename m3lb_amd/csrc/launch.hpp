@@ -24,6 +24,10 @@ struct KernelSet {
   // regularised walls: corner-density pass (both halves) and the wall update
   void (*wall_density)(const WallArgs&, int parity, hipStream_t);
   void (*walls)(const WallArgs&, int parity, hipStream_t);
+  // the neighbour sets wall nodes read, taken before the step
+  void (*snapshot)(const SnapArgs&, int parity, hipStream_t);
+  // specular ghosts and on-node specular nodes
+  void (*specular)(const SpecArgs&, int parity, hipStream_t);
   // batched server step; null for the identity op and 16-bit storage
   void (*batched)(const BatchArgs&, unsigned grid, hipStream_t);
   int elem_bytes;    // bytes per stored population
@@ -84,6 +88,18 @@ struct Launch {
       else k_walls<K, 1><<<g, kBlock, 0, s>>>(a);
     }
   }
+  static void snapshot(const SnapArgs& a, int parity, hipStream_t s) {
+    const unsigned g = blocks_for(a.count);
+    if (parity == 0) k_snapshot<K, 0><<<g, kBlock, 0, s>>>(a);
+    else k_snapshot<K, 1><<<g, kBlock, 0, s>>>(a);
+  }
+  static void specular(const SpecArgs& a, int parity, hipStream_t s) {
+    if constexpr (K::OP != M3LB_OP_NULL) {
+      const unsigned g = blocks_for(a.count);
+      if (parity == 0) k_specular<K, 0><<<g, kBlock, 0, s>>>(a);
+      else k_specular<K, 1><<<g, kBlock, 0, s>>>(a);
+    }
+  }
   static void exchange(const ReadArgs& a, const uint8_t* body, unsigned grid, hipStream_t s) {
     k_momentum_exchange<K><<<grid, kBlock, 0, s>>>(a, body);
   }
@@ -97,6 +113,7 @@ struct Launch {
     static const KernelSet t = {step,     seed,      gather, storage,
                                 macro,    reduce_mass, sum_exact, absmax,
                                 rescale,  exchange, wall_density, walls,
+                                snapshot, specular,
                                 kBatched ? batched : nullptr,
                                 int(sizeof(typename K::T)), int(sizeof(typename K::R))};
     return &t;

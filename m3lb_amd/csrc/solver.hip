@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -78,6 +79,43 @@ int validate(const m3lb_config* c) {
 
 }  // namespace
 
+// Boundary nodes set by their own calls (FluidSolver.hpp:193-530,
+// 1259-1300): regularised walls (per padded node: normal code, table tag,
+// unknown mask, corner axis, outflow donor and its face, closure density) and
+// specular cells (ghost normal, on-node face mask).  Host copies are kept to
+// build the per-step plan.
+struct Boundaries {
+  // regularised walls, host
+  std::vector<uint8_t> nrm, ext, onrm;
+  std::vector<uint16_t> tag;
+  std::vector<uint32_t> unk;
+  std::vector<unsigned> don;
+  std::vector<double> wtab, wtab0;  // (ux, uy, uz, rho) per state, as the compute type holds them
+  bool walls_set = false, has_corners = false, un_on = false;
+  int fd_corners = 1;  // fd_corners_ = has_shear_omega<Collision> (FluidSolver.hpp:1287)
+  int force_bc = 1;    // force_bc_ (:1288)
+  int out_order = 1;   // out_order_ (:1291)
+  double outflow_rho = 1.0, un_scale = 1.0;
+  // regularised walls, device
+  uint8_t *nrm_d = nullptr, *ext_d = nullptr, *onrm_d = nullptr;
+  uint16_t* tag_d = nullptr;
+  uint32_t* unk_d = nullptr;
+  unsigned* don_d = nullptr;
+  void* rho_d = nullptr;
+  double* wtab_d = nullptr;
+  void* un_d = nullptr;
+  // specular cells
+  std::vector<uint8_t> spec_nrm, spec_faces;
+  uint8_t *spec_nrm_d = nullptr, *spec_faces_d = nullptr;
+  // the per-step plan, rebuilt from the flags when they or the reads change
+  bool dirty = true;
+  int nwall = 0, nspec = 0, nsnap = 0;
+  std::vector<int> level_off;  // wall list offsets per level, then the end
+  unsigned *wlist_d = nullptr, *slist_d = nullptr, *snap_nodes_d = nullptr;
+  int* rd_d = nullptr;
+  void* snap_d = nullptr;
+};
+
 struct m3lb_solver {
   m3lb_config cfg;
   int Q = 0;
@@ -101,17 +139,7 @@ struct m3lb_solver {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int remap = 64;  // step-kernel tile order (kernels.hpp tile_of)
   void* ff_d = nullptr;  // M3LB_FORCE_FIELD: fx, fy, fz planes of `stride` (compute type)
-  // regularised walls (m3lb_set_regularized_walls)
-  int nwall = 0;
-  bool has_corners = false;
-  int fd_corners = 1;  // fd_corners_ = has_shear_omega<Collision> (FluidSolver.hpp:1287)
-  uint8_t* nrm_d = nullptr;
-  uint16_t* tag_d = nullptr;
-  uint32_t* unk_d = nullptr;
-  uint8_t* ext_d = nullptr;
-  void* wrho_d = nullptr;
-  double* wtab_d = nullptr;
-  unsigned* wlist_d = nullptr;
+  Boundaries b;  // regularised walls and specular cells
 };
 
 namespace {
@@ -164,31 +192,48 @@ m3lb::ReadArgs read_args(const m3lb_solver* s) {
   for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
   a.ff = s->ff_d;
   for (int q = 0; q < 27; ++q) a.inv_scale[q] = s->inv_scale[q];
-  if (s->nwall) {
-    a.bc_nrm = s->nrm_d;
-    a.bc_tag = s->tag_d;
-    a.bc_rho = s->wrho_d;
-    a.wtab = s->wtab_d;
+  const Boundaries& b = s->b;
+  if (b.walls_set) {
+    a.bc_nrm = b.nrm_d;
+    a.bc_tag = b.tag_d;
+    a.bc_rho = b.rho_d;
+    a.wtab = b.wtab_d;
+    a.bc_don = b.don_d;
+    a.bc_onrm = b.onrm_d;
+    a.wall_un = b.un_on ? b.un_d : nullptr;
+    a.un_scale = b.un_scale;
+    a.outflow_rho = b.outflow_rho;
+    a.out_order = b.out_order;
   }
+  a.spec_faces = b.spec_faces_d;
   return a;
 }
 
 m3lb::WallArgs wall_args(const m3lb_solver* s) {
+  const Boundaries& b = s->b;
   m3lb::WallArgs a;
   std::memset(&a, 0, sizeof a);
   a.f = s->f;
   a.stride = s->stride;
   a.flags = s->flags_d;
   a.d = s->d;
-  a.list = s->wlist_d;
-  a.nwall = s->nwall;
-  a.fd_corners = s->fd_corners;
-  a.bc_nrm = s->nrm_d;
-  a.bc_tag = s->tag_d;
-  a.bc_unk = s->unk_d;
-  a.bc_ext = s->ext_d;
-  a.bc_rho = s->wrho_d;
-  a.wtab = s->wtab_d;
+  a.list = b.wlist_d;
+  a.nwall = b.nwall;
+  a.fd_corners = b.fd_corners;
+  a.bc_nrm = b.nrm_d;
+  a.bc_tag = b.tag_d;
+  a.bc_unk = b.unk_d;
+  a.bc_ext = b.ext_d;
+  a.bc_rho = b.rho_d;
+  a.wtab = b.wtab_d;
+  a.bc_don = b.don_d;
+  a.wall_un = b.un_on ? b.un_d : nullptr;
+  a.un_scale = b.un_scale;
+  a.outflow_rho = b.outflow_rho;
+  a.out_order = b.out_order;
+  a.force_bc = b.force_bc;
+  a.rd = b.rd_d;
+  a.snap = b.snap_d;
   a.omega = s->cfg.omega;
   a.omega_m = s->cfg.omega_minus;
   a.omega_b = s->cfg.omega_bulk;
@@ -198,14 +243,181 @@ m3lb::WallArgs wall_args(const m3lb_solver* s) {
   return a;
 }
 
-void free_walls(m3lb_solver* s) {
-  void* ps[] = {s->nrm_d, s->tag_d, s->unk_d, s->ext_d, s->wrho_d, s->wtab_d, s->wlist_d};
-  for (void* p : ps)
-    if (p) (void)hipFree(p);
-  s->nrm_d = nullptr; s->tag_d = nullptr; s->unk_d = nullptr; s->ext_d = nullptr;
-  s->wrho_d = nullptr; s->wtab_d = nullptr; s->wlist_d = nullptr;
-  s->nwall = 0;
-  s->has_corners = false;
+m3lb::SpecArgs spec_args(const m3lb_solver* s) {
+  const Boundaries& b = s->b;
+  m3lb::SpecArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.f = s->f;
+  a.stride = s->stride;
+  a.flags = s->flags_d;
+  a.d = s->d;
+  a.list = b.slist_d;
+  a.count = b.nspec;
+  a.spec_nrm = b.spec_nrm_d;
+  a.spec_faces = b.spec_faces_d;
+  a.omega = s->cfg.omega;
+  a.omega_m = s->cfg.omega_minus;
+  a.omega_b = s->cfg.omega_bulk;
+  for (int k = 0; k < 3; ++k) a.F[k] = s->cfg.force[k];
+  a.ff = s->ff_d;
+  for (int q = 0; q < 27; ++q) { a.scale[q] = s->scale[q]; a.inv_scale[q] = s->inv_scale[q]; }
+  return a;
+}
+
+template <class P>
+void free_dev(P*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_boundaries(m3lb_solver* s) {
+  Boundaries& b = s->b;
+  free_dev(b.nrm_d); free_dev(b.ext_d); free_dev(b.onrm_d); free_dev(b.tag_d);
+  free_dev(b.unk_d); free_dev(b.don_d); free_dev(b.rho_d); free_dev(b.wtab_d);
+  free_dev(b.un_d); free_dev(b.spec_nrm_d); free_dev(b.spec_faces_d); free_dev(b.wlist_d);
+  free_dev(b.slist_d); free_dev(b.snap_nodes_d); free_dev(b.rd_d); free_dev(b.snap_d);
+}
+
+// A value as the compute type holds it (the reference's tables are Real).
+double as_real(const m3lb_solver* s, double v) {
+  return s->ks->real_bytes == 8 ? v : double(float(v));
+}
+double real_mul(const m3lb_solver* s, double a, double c) {
+  return s->ks->real_bytes == 8 ? a * c : double(float(a) * float(c));
+}
+
+// Re-uploads a small host vector into a device array of its size.
+template <class T>
+int upload(T*& dst, const std::vector<T>& src, hipStream_t st) {
+  free_dev(dst);
+  HIP_OK(hipMalloc(&dst, std::max<size_t>(1, src.size()) * sizeof(T)));
+  if (!src.empty())
+    HIP_OK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  return M3LB_OK;
+}
+
+// The per-step plan.  The reference's results were made with its serial
+// execution space, which sweeps the nodes in ascending order in one pass; a
+// wall node that reads a NEIGHBOUR's slots (an outlet's upstream nodes, an
+// outflow donor, a corner's finite-difference stencil) then sees an earlier
+// neighbour after that neighbour's update (its post-collision set, pair-swapped
+// in the slots) and a later one before.  Here every wall node runs after the
+// fluid and specular kernels, in levels so that a wall read by a later wall
+// runs first, and the sets of later neighbours are copied before the step
+// (k_snapshot); rd says which source each read uses.  Nodes that read no
+// neighbour are unaffected by order.
+int rebuild_plan(m3lb_solver* s) {
+  Boundaries& b = s->b;
+  const m3lb::Dom& d = s->d;
+  const size_t np = size_t(d.np);
+  std::vector<unsigned> walls, spec;
+  for (size_t n = 0; n < np; ++n) {
+    const uint8_t f = s->flags_h[n];
+    if (f == M3LB_REGWALL) walls.push_back(unsigned(n));
+    else if (f == M3LB_SPECWALL || f == M3LB_SPECNODE) spec.push_back(unsigned(n));
+  }
+  auto coords = [&](unsigned n, int& x, int& y, int& z) {
+    x = int(n % unsigned(d.sx)) - d.hx;
+    y = int((n / unsigned(d.sx)) % unsigned(d.sy)) - d.hy;
+    z = int(n / unsigned(d.sx * d.sy)) - d.hz;
+  };
+  auto id = [&](int x, int y, int z) { return unsigned(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx); };
+  const int D = s->Q == 9 ? 2 : 3;
+  std::vector<int> level(np, 0);
+  std::vector<std::array<int64_t, m3lb::kWallReads>> reads(walls.size());
+  int maxlev = 0;
+  for (size_t w = 0; w < walls.size(); ++w) {
+    const unsigned n = walls[w];
+    auto& r = reads[w];
+    r.fill(-1);  // -1: no read
+    const int code = b.nrm[n];
+    int x, y, z;
+    coords(n, x, y, z);
+    if (code == m3lb::kNrmOutXp || code == m3lb::kNrmOutEq) {
+      r[0] = id(x - 1, y, z);
+      if (code == m3lb::kNrmOutXp && b.out_order >= 2) r[1] = id(x - 2, y, z);
+    } else if (code == m3lb::kNrmOutFree) {
+      r[0] = b.don[n];
+    } else if (code == m3lb::kNrmCorner && b.fd_corners) {
+      const int ic[3] = {x, y, z}, nn[3] = {d.nx, d.ny, d.nz}, per[3] = {d.px, d.py, d.pz};
+      for (int ax = 0; ax < D; ++ax) {
+        if (nn[ax] < 3) continue;
+        int dir = 0;
+        if (!per[ax]) {
+          if (ic[ax] == 0) dir = 1;
+          else if (ic[ax] == nn[ax] - 1) dir = -1;
+        }
+        int c1[3] = {ic[0], ic[1], ic[2]}, c2[3] = {ic[0], ic[1], ic[2]};
+        if (dir != 0) {
+          c1[ax] += dir; c2[ax] += 2 * dir;
+        } else {
+          c1[ax] = (ic[ax] + 1) % nn[ax];
+          c2[ax] = (ic[ax] - 1 + nn[ax]) % nn[ax];
+        }
+        const unsigned m1 = id(c1[0], c1[1], c1[2]), m2 = id(c2[0], c2[1], c2[2]);
+        if (s->flags_h[m1] != M3LB_REGWALL) r[2 * ax] = m1;  // walls: table velocity
+        if (s->flags_h[m2] != M3LB_REGWALL) r[2 * ax + 1] = m2;
+      }
+    }
+    for (int k = 0; k < m3lb::kWallReads; ++k)
+      if (r[k] >= 0 && unsigned(r[k]) < n && s->flags_h[size_t(r[k])] == M3LB_REGWALL)
+        level[n] = std::max(level[n], level[size_t(r[k])] + 1);
+    maxlev = std::max(maxlev, level[n]);
+  }
+  // wall list by (level, node); rd rows in the same order
+  std::vector<size_t> order(walls.size());
+  for (size_t w = 0; w < walls.size(); ++w) order[w] = w;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t c) { return level[walls[a]] < level[walls[c]]; });
+  std::vector<unsigned> wlist(walls.size()), snap_nodes;
+  std::vector<int> rd(walls.size() * m3lb::kWallReads, -1);
+  b.level_off.assign(size_t(maxlev) + 2, int(walls.size()));
+  for (size_t k = 0; k < order.size(); ++k) {
+    const size_t w = order[k];
+    const unsigned n = walls[w];
+    wlist[k] = n;
+    const int lv = level[n];
+    if (b.level_off[size_t(lv)] > int(k)) b.level_off[size_t(lv)] = int(k);
+    for (int r = 0; r < m3lb::kWallReads; ++r) {
+      const int64_t m = reads[w][size_t(r)];
+      if (m > int64_t(n)) {  // a later node: its set from before the step
+        rd[k * m3lb::kWallReads + size_t(r)] = int(snap_nodes.size());
+        snap_nodes.push_back(unsigned(m));
+      }
+    }
+  }
+  for (int lv = maxlev; lv >= 0; --lv)  // levels are contiguous; fill any gap
+    b.level_off[size_t(lv)] = std::min(b.level_off[size_t(lv)], b.level_off[size_t(lv) + 1]);
+  b.nwall = int(walls.size());
+  b.nspec = int(spec.size());
+  b.nsnap = int(snap_nodes.size());
+  int rc = upload(b.wlist_d, wlist, s->stream);
+  if (!rc) rc = upload(b.slist_d, spec, s->stream);
+  if (!rc) rc = upload(b.snap_nodes_d, snap_nodes, s->stream);
+  if (!rc) rc = upload(b.rd_d, rd, s->stream);
+  if (rc) return rc;
+  free_dev(b.snap_d);
+  HIP_OK(hipMalloc(&b.snap_d, std::max<size_t>(1, snap_nodes.size()) * size_t(s->Q) *
+                                  size_t(s->ks->real_bytes)));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  b.dirty = false;
+  return M3LB_OK;
+}
+
+bool any_boundary_flag(const m3lb_solver* s) {
+  for (uint8_t f : s->flags_h)
+    if (f == M3LB_REGWALL || f == M3LB_SPECWALL || f == M3LB_SPECNODE) return true;
+  return false;
+}
+
+void refresh_geometry(m3lb_solver* s) {
+  const m3lb::Dom& d = s->d;
+  bool geom = false;
+  for (int z = 0; z < d.nz && !geom; ++z)
+    for (int y = 0; y < d.ny && !geom; ++y)
+      for (int x = 0; x < d.nx && !geom; ++x)
+        geom = s->flags_h[size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx)] != M3LB_FLUID;
+  s->geometry = geom;
 }
 
 m3lb::SeedArgs seed_args(const m3lb_solver* s) {
@@ -380,7 +592,7 @@ int m3lb_destroy(m3lb_solver* s) {
   if (s->f) hipFree(s->f);
   if (s->flags_d) hipFree(s->flags_d);
   if (s->ff_d) hipFree(s->ff_d);
-  free_walls(s);
+  free_boundaries(s);
   for (void* p : s->macro_d)
     if (p) hipFree(p);
   if (s->partial_d) hipFree(s->partial_d);
@@ -393,23 +605,20 @@ int m3lb_destroy(m3lb_solver* s) {
 
 int m3lb_set_geometry(m3lb_solver* s, const uint8_t* fl) {
   if (!s || !fl) return fail(M3LB_EINVAL, "null argument");
-  // every interior flag is rewritten, so regularised walls set earlier are gone
-  if (s->nwall) {
-    HIP_OK(hipStreamSynchronize(s->stream));
-    free_walls(s);
-  }
   const m3lb::Dom& d = s->d;
-  bool geom = false;
+  const size_t ni = size_t(d.nx) * size_t(d.ny) * size_t(d.nz);
+  for (size_t k = 0; k < ni; ++k)
+    if (fl[k] > M3LB_EXCLUDED) return fail(M3LB_EINVAL, "flag values are Fluid(0), Solid(1), Excluded(2)");
+  // every interior flag is rewritten, so wall and specular nodes set earlier
+  // are gone (their per-node data stays, unread)
+  HIP_OK(hipStreamSynchronize(s->stream));
   size_t k = 0;
   for (int z = 0; z < d.nz; ++z)
     for (int y = 0; y < d.ny; ++y)
-      for (int x = 0; x < d.nx; ++x, ++k) {
-        const uint8_t v = fl[k];
-        if (v > M3LB_EXCLUDED) return fail(M3LB_EINVAL, "flag values are Fluid(0), Solid(1), Excluded(2)");
-        s->flags_h[size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx)] = v;
-        geom = geom || v != M3LB_FLUID;
-      }
-  s->geometry = geom;
+      for (int x = 0; x < d.nx; ++x, ++k)
+        s->flags_h[size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx)] = fl[k];
+  refresh_geometry(s);
+  s->b.dirty = true;
   HIP_OK(hipMemcpyAsync(s->flags_d, s->flags_h.data(), s->flags_h.size(), hipMemcpyHostToDevice, s->stream));
   return sync(s);
 }
@@ -490,18 +699,47 @@ int m3lb_step(m3lb_solver* s, int32_t nsteps, int32_t store_macroscopic) {
   const unsigned rows_per_block = unsigned(m3lb::kBlock >> a.bx_log2);
   const unsigned grid = a.chunks > 1 ? unsigned(a.rows) * unsigned(a.chunks)
                                      : unsigned((a.rows + rows_per_block - 1) / rows_per_block);
-  const m3lb::WallArgs wa = s->nwall ? wall_args(s) : m3lb::WallArgs{};
+  Boundaries& b = s->b;
+  if (b.dirty) {
+    const int rc = rebuild_plan(s);
+    if (rc) return rc;
+  }
+  m3lb::WallArgs wa = wall_args(s);
+  m3lb::SpecArgs sa = spec_args(s);
+  m3lb::SnapArgs na;
+  std::memset(&na, 0, sizeof na);
+  na.f = s->f;
+  na.stride = s->stride;
+  na.d = s->d;
+  na.nodes = b.snap_nodes_d;
+  na.count = b.nsnap;
+  na.out = b.snap_d;
+  for (int q = 0; q < 27; ++q) na.inv_scale[q] = s->inv_scale[q];
   for (int k = 0; k < nsteps; ++k) {
     const bool last = k == nsteps - 1;
-    for (int j = 0; j < 4; ++j) a.macro[j] = (store_macroscopic && last) ? s->macro_d[j] : nullptr;
+    for (int j = 0; j < 4; ++j) {
+      a.macro[j] = (store_macroscopic && last) ? s->macro_d[j] : nullptr;
+      wa.macro[j] = sa.macro[j] = a.macro[j];
+    }
     const int parity = int(s->t % 2);
-    if (s->nwall) {
-      // FluidSolver::step (:583): the corner-density pass, then the step; the
-      // wall nodes' share of the step runs first (see kernels.hpp, k_walls)
-      if (s->has_corners) s->ks->wall_density(wa, parity, s->stream);
+    // FluidSolver::step (:582-591): the corner-density pass, then one sweep
+    // in the reference's node order (see rebuild_plan)
+    if (b.nwall && b.has_corners) {
+      wa.list = b.wlist_d;
+      wa.nwall = b.nwall;
+      s->ks->wall_density(wa, parity, s->stream);
+    }
+    if (b.nsnap) s->ks->snapshot(na, parity, s->stream);
+    if (b.nspec) s->ks->specular(sa, parity, s->stream);
+    s->ks->step(a, parity, s->geometry, grid, s->stream);
+    for (size_t lv = 0; lv + 1 < b.level_off.size(); ++lv) {
+      const int lo = b.level_off[lv], hi = b.level_off[lv + 1];
+      if (hi <= lo) continue;
+      wa.list = b.wlist_d + lo;
+      wa.rd = b.rd_d + size_t(lo) * m3lb::kWallReads;
+      wa.nwall = hi - lo;
       s->ks->walls(wa, parity, s->stream);
     }
-    s->ks->step(a, parity, s->geometry, grid, s->stream);
     ++s->t;
   }
   s->macro_valid = store_macroscopic != 0;
@@ -555,9 +793,15 @@ int m3lb_macroscopic_stored(m3lb_solver* s, double* rho, double* ux, double* uy,
     for (size_t n = 0; n < np; ++n)
       dst[k][n] = rb == 8 ? reinterpret_cast<const double*>(tmp.data())[n]
                           : double(reinterpret_cast<const float*>(tmp.data())[n]);
-    // non-fluid nodes: the step never writes them; report 0 as compute_macroscopic does
-    for (size_t n = 0; n < np; ++n)
-      if (s->flags_h[n] != M3LB_FLUID) dst[k][n] = 0.0;
+    // nodes the step does not collide (Solid, Excluded, specular ghosts and
+    // the population-extrapolation and equilibrium outlets): report 0
+    for (size_t n = 0; n < np; ++n) {
+      const uint8_t f = s->flags_h[n];
+      bool stored = f == M3LB_FLUID || f == M3LB_SPECNODE;
+      if (f == M3LB_REGWALL)
+        stored = s->b.nrm[n] != m3lb::kNrmOutFree && s->b.nrm[n] != m3lb::kNrmOutEq;
+      if (!stored) dst[k][n] = 0.0;
+    }
   }
   return M3LB_OK;
 }
@@ -827,18 +1071,37 @@ int m3lb_set_regularized_walls(m3lb_solver* s, const uint8_t* normals, const uin
   const m3lb::Dom& d = s->d;
   const size_t np = size_t(d.np);
   auto id = [&](int x, int y, int z) { return size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx); };
-  std::vector<uint8_t> nrm(np, 0), ext(np, 0);
-  std::vector<uint16_t> tag(np, 0);
-  std::vector<uint32_t> unk(np, 0);
+  // validate everything before changing anything
+  {
+    size_t k = 0;
+    for (int z = 0; z < d.nz; ++z)
+      for (int y = 0; y < d.ny; ++y)
+        for (int x = 0; x < d.nx; ++x, ++k) {
+          const int c = normals[k];
+          if (c > m3lb::kNrmOutEq) return fail(M3LB_EINVAL, "normal codes are 0..10");
+          if (c && tags[k] >= ntab) return fail(M3LB_EINVAL, "wall tag outside the table");
+          // the outlets read their upstream nodes along -x (two for order 2)
+          if ((c == m3lb::kNrmOutXp && x < 2) || (c == m3lb::kNrmOutEq && x < 1))
+            return fail(M3LB_EINVAL, "an outflow node needs upstream nodes along -x");
+        }
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  Boundaries& b = s->b;
+  b.nrm.assign(np, 0);
+  b.tag.assign(np, 0);
+  b.ext.assign(np, 0);
+  b.unk.assign(np, 0);
+  b.onrm.assign(np, 0);
+  b.don.resize(np);
+  for (size_t n = 0; n < np; ++n) b.don[n] = unsigned(n);
   size_t k = 0;
   for (int z = 0; z < d.nz; ++z)
     for (int y = 0; y < d.ny; ++y)
       for (int x = 0; x < d.nx; ++x, ++k) {
-        if (normals[k] > m3lb::kNrmCorner) return fail(M3LB_EINVAL, "normal codes are 0..7");
         if (normals[k] == 0) continue;
-        if (tags[k] >= ntab) return fail(M3LB_EINVAL, "wall tag outside the table");
-        nrm[id(x, y, z)] = normals[k];
-        tag[id(x, y, z)] = tags[k];
+        b.nrm[id(x, y, z)] = normals[k];
+        b.tag[id(x, y, z)] = tags[k];
+        s->flags_h[id(x, y, z)] = M3LB_REGWALL;
       }
   // corners: an axis direction whose next two nodes are straight-wall nodes
   static const int dirs[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
@@ -846,23 +1109,20 @@ int m3lb_set_regularized_walls(m3lb_solver* s, const uint8_t* normals, const uin
   for (int z = 0; z < d.nz; ++z)
     for (int y = 0; y < d.ny; ++y)
       for (int x = 0; x < d.nx; ++x) {
-        if (nrm[id(x, y, z)] != m3lb::kNrmCorner) continue;
+        if (b.nrm[id(x, y, z)] != m3lb::kNrmCorner) continue;
         corners = true;
         for (int q = 0; q < (d.nz > 1 ? 6 : 4); ++q) {
           const int x1 = x + dirs[q][0], y1 = y + dirs[q][1], z1 = z + dirs[q][2];
           const int x2 = x + 2 * dirs[q][0], y2 = y + 2 * dirs[q][1], z2 = z + 2 * dirs[q][2];
           if (x1 < 0 || y1 < 0 || z1 < 0 || x2 < 0 || y2 < 0 || z2 < 0) continue;
           if (x1 >= d.nx || y1 >= d.ny || z1 >= d.nz || x2 >= d.nx || y2 >= d.ny || z2 >= d.nz) continue;
-          const int a = nrm[id(x1, y1, z1)], b = nrm[id(x2, y2, z2)];
-          if (a != 0 && a != m3lb::kNrmCorner && b != 0 && b != m3lb::kNrmCorner) {
-            ext[id(x, y, z)] = uint8_t(q + 1);
+          const int a = b.nrm[id(x1, y1, z1)], c = b.nrm[id(x2, y2, z2)];
+          if (a != 0 && a != m3lb::kNrmCorner && c != 0 && c != m3lb::kNrmCorner) {
+            b.ext[id(x, y, z)] = uint8_t(q + 1);
             break;
           }
         }
       }
-  // flags: the wall nodes become RegWall
-  for (size_t n = 0; n < np; ++n)
-    if (nrm[n]) s->flags_h[n] = M3LB_REGWALL;
   // unknown directions: source p - c_i outside a non-periodic extent, or an
   // Excluded or Solid node (FluidSolver.hpp:420-455)
   const int next[3] = {d.nx, d.ny, d.nz}, per[3] = {d.px, d.py, d.pz};
@@ -870,7 +1130,7 @@ int m3lb_set_regularized_walls(m3lb_solver* s, const uint8_t* normals, const uin
     for (int y = 0; y < d.ny; ++y)
       for (int x = 0; x < d.nx; ++x) {
         const size_t n = id(x, y, z);
-        if (!nrm[n]) continue;
+        if (!b.nrm[n]) continue;
         uint32_t m = 0;
         for (int i = 0; i < s->Q; ++i) {
           const int c[3] = {s->Q == 9 ? m3lb::Lat<9>::cx[i] : m3lb::Lat<27>::cx[i],
@@ -888,47 +1148,209 @@ int m3lb_set_regularized_walls(m3lb_solver* s, const uint8_t* normals, const uin
           }
           if (outside) m |= 1u << i;
         }
-        unk[n] = m;
+        b.unk[n] = m;
       }
-  std::vector<unsigned> list;
-  for (size_t n = 0; n < np; ++n)
-    if (nrm[n]) list.push_back(unsigned(n));
-  std::vector<double> tab(size_t(ntab) * 4);
-  for (int t = 0; t < ntab; ++t) {
-    tab[size_t(t) * 4 + 0] = table[t].ux;
-    tab[size_t(t) * 4 + 1] = table[t].uy;
-    tab[size_t(t) * 4 + 2] = table[t].uz;
-    tab[size_t(t) * 4 + 3] = table[t].rho;
+  // NrmOutFree donors (:475-520): the first axis step that leaves the fluid
+  // (outside, or Solid) whose opposite neighbour is Fluid; none: the node itself
+  {
+    static const int dd[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    static const uint8_t codes[6] = {2, 1, 4, 3, 6, 5};
+    for (int z = 0; z < d.nz; ++z)
+      for (int y = 0; y < d.ny; ++y)
+        for (int x = 0; x < d.nx; ++x) {
+          const size_t n = id(x, y, z);
+          if (b.nrm[n] != m3lb::kNrmOutFree) continue;
+          for (int q = 0; q < 6; ++q) {
+            const int qx = x + dd[q][0], qy = y + dd[q][1], qz = z + dd[q][2];
+            const bool leaves = qx < 0 || qx >= d.nx || qy < 0 || qy >= d.ny || qz < 0 ||
+                                qz >= d.nz || s->flags_h[id(qx, qy, qz)] == M3LB_SOLID;
+            if (!leaves) continue;
+            const int o = q ^ 1;
+            const int rx = x + dd[o][0], ry = y + dd[o][1], rz = z + dd[o][2];
+            if (rx < 0 || rx >= d.nx || ry < 0 || ry >= d.ny || rz < 0 || rz >= d.nz) continue;
+            const size_t m = id(rx, ry, rz);
+            if (s->flags_h[m] != M3LB_FLUID) continue;
+            b.don[n] = unsigned(m);
+            b.onrm[n] = codes[q];
+            break;
+          }
+        }
   }
-  HIP_OK(hipStreamSynchronize(s->stream));
-  free_walls(s);
-  HIP_OK(hipMalloc(&s->nrm_d, np));
-  HIP_OK(hipMalloc(&s->tag_d, np * sizeof(uint16_t)));
-  HIP_OK(hipMalloc(&s->unk_d, np * sizeof(uint32_t)));
-  HIP_OK(hipMalloc(&s->ext_d, np));
-  HIP_OK(hipMalloc(&s->wrho_d, np * size_t(s->ks->real_bytes)));
-  HIP_OK(hipMalloc(&s->wtab_d, tab.size() * sizeof(double)));
-  HIP_OK(hipMalloc(&s->wlist_d, std::max<size_t>(1, list.size()) * sizeof(unsigned)));
-  HIP_OK(hipMemcpy(s->nrm_d, nrm.data(), np, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(s->tag_d, tag.data(), np * sizeof(uint16_t), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(s->unk_d, unk.data(), np * sizeof(uint32_t), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(s->ext_d, ext.data(), np, hipMemcpyHostToDevice));
-  HIP_OK(hipMemset(s->wrho_d, 0, np * size_t(s->ks->real_bytes)));
-  HIP_OK(hipMemcpy(s->wtab_d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-  if (!list.empty())
-    HIP_OK(hipMemcpy(s->wlist_d, list.data(), list.size() * sizeof(unsigned), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(s->flags_d, s->flags_h.data(), np, hipMemcpyHostToDevice));
-  s->nwall = int(list.size());
-  s->has_corners = corners;
-  s->geometry = s->geometry || s->nwall > 0;
+  b.wtab.resize(size_t(ntab) * 4);
+  for (int t = 0; t < ntab; ++t) {
+    const double v[4] = {table[t].ux, table[t].uy, table[t].uz, table[t].rho};
+    for (int a = 0; a < 4; ++a) b.wtab[size_t(t) * 4 + size_t(a)] = as_real(s, v[a]);
+  }
+  b.wtab0 = b.wtab;
+  b.has_corners = corners;
+  int rc = upload(b.nrm_d, b.nrm, s->stream);
+  if (!rc) rc = upload(b.tag_d, b.tag, s->stream);
+  if (!rc) rc = upload(b.unk_d, b.unk, s->stream);
+  if (!rc) rc = upload(b.ext_d, b.ext, s->stream);
+  if (!rc) rc = upload(b.onrm_d, b.onrm, s->stream);
+  if (!rc) rc = upload(b.don_d, b.don, s->stream);
+  if (!rc) rc = upload(b.wtab_d, b.wtab, s->stream);
+  if (rc) return rc;
+  if (!b.rho_d) {
+    HIP_OK(hipMalloc(&b.rho_d, np * size_t(s->ks->real_bytes)));
+    HIP_OK(hipMemsetAsync(b.rho_d, 0, np * size_t(s->ks->real_bytes), s->stream));
+  }
+  HIP_OK(hipMemcpyAsync(s->flags_d, s->flags_h.data(), np, hipMemcpyHostToDevice, s->stream));
+  b.walls_set = true;
+  b.dirty = true;
+  refresh_geometry(s);
   s->macro_valid = false;
-  return M3LB_OK;
+  return sync(s);
 }
 
 int m3lb_set_fd_corners(m3lb_solver* s, int32_t on) {
   if (!s) return fail(M3LB_EINVAL, "null handle");
-  s->fd_corners = on ? 1 : 0;
+  s->b.fd_corners = on ? 1 : 0;
+  s->b.dirty = true;
   return M3LB_OK;
+}
+
+int m3lb_set_force_correction(m3lb_solver* s, int32_t on) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  s->b.force_bc = on ? 1 : 0;
+  return M3LB_OK;
+}
+
+int m3lb_set_outflow_order(m3lb_solver* s, int32_t order) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  s->b.out_order = order;
+  s->b.dirty = true;
+  return M3LB_OK;
+}
+
+int m3lb_set_outflow_density(m3lb_solver* s, double rho) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  if (!std::isfinite(rho)) return fail(M3LB_EINVAL, "density must be finite");
+  s->b.outflow_rho = as_real(s, rho);
+  return M3LB_OK;
+}
+
+int m3lb_set_wall_velocity_scale(m3lb_solver* s, double scale) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  Boundaries& b = s->b;
+  if (!b.walls_set) return fail(M3LB_ESTATE, "no regularised walls are set");
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (size_t t = 0; t + 3 < b.wtab.size(); t += 4)
+    for (size_t a = 0; a < 3; ++a) b.wtab[t + a] = real_mul(s, b.wtab0[t + a], scale);
+  HIP_OK(hipMemcpy(b.wtab_d, b.wtab.data(), b.wtab.size() * sizeof(double), hipMemcpyHostToDevice));
+  return M3LB_OK;
+}
+
+int m3lb_set_wall_density(m3lb_solver* s, double rho) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  Boundaries& b = s->b;
+  if (!b.walls_set) return fail(M3LB_ESTATE, "no regularised walls are set");
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (size_t t = 0; t + 3 < b.wtab.size(); t += 4) b.wtab[t + 3] = b.wtab0[t + 3] = as_real(s, rho);
+  HIP_OK(hipMemcpy(b.wtab_d, b.wtab.data(), b.wtab.size() * sizeof(double), hipMemcpyHostToDevice));
+  return M3LB_OK;
+}
+
+int m3lb_set_wall_velocity_field(m3lb_solver* s, const double* ux, const double* uy, const double* uz) {
+  if (!s || !ux || !uy || !uz) return fail(M3LB_EINVAL, "null argument");
+  const m3lb::Dom& d = s->d;
+  const size_t np = size_t(d.np), rb = size_t(s->ks->real_bytes);
+  std::vector<char> h(3 * np * rb, 0);
+  size_t k = 0;
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x, ++k) {
+        const size_t n = size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx);
+        const double v[3] = {ux[k], uy[k], uz[k]};
+        for (int a = 0; a < 3; ++a) {
+          if (rb == 8) reinterpret_cast<double*>(h.data())[size_t(a) * np + n] = v[a];
+          else reinterpret_cast<float*>(h.data())[size_t(a) * np + n] = float(v[a]);
+        }
+      }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  Boundaries& b = s->b;
+  if (!b.un_d) HIP_OK(hipMalloc(&b.un_d, h.size()));
+  HIP_OK(hipMemcpy(b.un_d, h.data(), h.size(), hipMemcpyHostToDevice));
+  b.un_on = true;
+  b.un_scale = 1.0;
+  return M3LB_OK;
+}
+
+int m3lb_set_wall_velocity_field_scale(m3lb_solver* s, double scale) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  s->b.un_scale = as_real(s, scale);
+  return M3LB_OK;
+}
+
+int m3lb_set_specular_walls(m3lb_solver* s, const uint8_t* normals) {
+  if (!s || !normals) return fail(M3LB_EINVAL, "null argument");
+  if (s->cfg.op == M3LB_OP_NULL) return fail(M3LB_EINVAL, "the identity collision has no walls");
+  const m3lb::Dom& d = s->d;
+  const size_t np = size_t(d.np), ni = size_t(d.nx) * size_t(d.ny) * size_t(d.nz);
+  for (size_t k = 0; k < ni; ++k)
+    if (normals[k] > M3LB_NRM_ZM)
+      return fail(M3LB_EINVAL, "specular reflection is defined for the axis normals 1..6 only");
+  HIP_OK(hipStreamSynchronize(s->stream));
+  Boundaries& b = s->b;
+  b.spec_nrm.assign(np, 0);
+  size_t k = 0;
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x, ++k) {
+        if (!normals[k]) continue;
+        const size_t n = size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx);
+        b.spec_nrm[n] = normals[k];
+        s->flags_h[n] = M3LB_SPECWALL;
+      }
+  int rc = upload(b.spec_nrm_d, b.spec_nrm, s->stream);
+  if (rc) return rc;
+  if (!b.spec_faces_d) {
+    b.spec_faces.assign(np, 0);
+    rc = upload(b.spec_faces_d, b.spec_faces, s->stream);
+    if (rc) return rc;
+  }
+  HIP_OK(hipMemcpyAsync(s->flags_d, s->flags_h.data(), np, hipMemcpyHostToDevice, s->stream));
+  b.dirty = true;
+  refresh_geometry(s);
+  s->macro_valid = false;
+  return sync(s);
+}
+
+int m3lb_set_specular_nodes(m3lb_solver* s, const uint8_t* faces) {
+  if (!s || !faces) return fail(M3LB_EINVAL, "null argument");
+  if (s->cfg.op == M3LB_OP_NULL) return fail(M3LB_EINVAL, "the identity collision has no walls");
+  const m3lb::Dom& d = s->d;
+  const size_t np = size_t(d.np), ni = size_t(d.nx) * size_t(d.ny) * size_t(d.nz);
+  for (size_t k = 0; k < ni; ++k) {
+    if (faces[k] > 0x3F) return fail(M3LB_EINVAL, "face masks use bits 0..5 only");
+    for (int a = 0; a < 3; ++a)
+      if ((faces[k] & (1u << (2 * a))) && (faces[k] & (1u << (2 * a + 1))))
+        return fail(M3LB_EINVAL, "a face mask carries both faces of one axis");
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  Boundaries& b = s->b;
+  b.spec_faces.assign(np, 0);
+  size_t k = 0;
+  for (int z = 0; z < d.nz; ++z)
+    for (int y = 0; y < d.ny; ++y)
+      for (int x = 0; x < d.nx; ++x, ++k) {
+        if (!faces[k]) continue;
+        const size_t n = size_t(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx);
+        b.spec_faces[n] = faces[k];
+        s->flags_h[n] = M3LB_SPECNODE;
+      }
+  int rc = upload(b.spec_faces_d, b.spec_faces, s->stream);
+  if (rc) return rc;
+  if (!b.spec_nrm_d) {
+    b.spec_nrm.assign(np, 0);
+    rc = upload(b.spec_nrm_d, b.spec_nrm, s->stream);
+    if (rc) return rc;
+  }
+  HIP_OK(hipMemcpyAsync(s->flags_d, s->flags_h.data(), np, hipMemcpyHostToDevice, s->stream));
+  b.dirty = true;
+  refresh_geometry(s);
+  s->macro_valid = false;
+  return sync(s);
 }
 
 int m3lb_set_force_field(m3lb_solver* s, const double* fx, const double* fy, const double* fz) {

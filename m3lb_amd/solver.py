@@ -47,6 +47,15 @@ SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_uint16), _dp,
                                                   ctypes.c_int32]),
     "m3lb_set_fd_corners": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "m3lb_set_force_correction": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "m3lb_set_outflow_order": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "m3lb_set_outflow_density": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m3lb_set_wall_velocity_scale": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m3lb_set_wall_density": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m3lb_set_wall_velocity_field": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "m3lb_set_wall_velocity_field_scale": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m3lb_set_specular_walls": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8)]),
+    "m3lb_set_specular_nodes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8)]),
     "m3lb_set_omega": (ctypes.c_int, [_vp] + [ctypes.c_double] * 3),
     "m3lb_get_scales": (ctypes.c_int, [_vp, _dp]),
     "m3lb_set_scales": (ctypes.c_int, [_vp, _dp]),
@@ -271,6 +280,41 @@ class FluidSolver:
 
     def set_fd_corners(self, on):
         _check(self.L.m3lb_set_fd_corners(self.h, 1 if on else 0))
+
+    def set_force_correction(self, on):
+        _check(self.L.m3lb_set_force_correction(self.h, 1 if on else 0))
+
+    def set_outflow_order(self, k):
+        _check(self.L.m3lb_set_outflow_order(self.h, int(k)))
+
+    def set_outflow_density(self, rho):
+        _check(self.L.m3lb_set_outflow_density(self.h, float(rho)))
+
+    def set_wall_velocity_scale(self, sc):
+        _check(self.L.m3lb_set_wall_velocity_scale(self.h, float(sc)))
+
+    def set_wall_density(self, rho):
+        _check(self.L.m3lb_set_wall_density(self.h, float(rho)))
+
+    def set_wall_velocity_field(self, ux, uy, uz):
+        """One wall velocity per interior node, x fastest."""
+        n = self.cfg.nx * self.cfg.ny * self.cfg.nz
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (ux, uy, uz)]
+        assert all(a.size == n for a in arrs)
+        _check(self.L.m3lb_set_wall_velocity_field(self.h, *[_d(a) for a in arrs]))
+
+    def set_wall_velocity_field_scale(self, sc):
+        _check(self.L.m3lb_set_wall_velocity_field_scale(self.h, float(sc)))
+
+    def set_specular_walls(self, normals):
+        a = np.ascontiguousarray(normals, dtype=np.uint8).reshape(-1)
+        assert a.size == self.cfg.nx * self.cfg.ny * self.cfg.nz
+        _check(self.L.m3lb_set_specular_walls(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+
+    def set_specular_nodes(self, faces):
+        a = np.ascontiguousarray(faces, dtype=np.uint8).reshape(-1)
+        assert a.size == self.cfg.nx * self.cfg.ny * self.cfg.nz
+        _check(self.L.m3lb_set_specular_nodes(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
 
     def set_force_field(self, fx, fy, fz):
         """m3lb_set_force_field: per-node Guo force, padded arrays."""

@@ -56,6 +56,15 @@ def load():
                                                      ctypes.POINTER(ctypes.c_uint16), _dp,
                                                      ctypes.c_int]),
         "orc_set_fd_corners": (None, [vp, ctypes.c_int]),
+        "orc_set_force_correction": (None, [vp, ctypes.c_int]),
+        "orc_set_outflow_order": (None, [vp, ctypes.c_int]),
+        "orc_set_outflow_density": (None, [vp, ctypes.c_double]),
+        "orc_set_wall_velocity_scale": (None, [vp, ctypes.c_double]),
+        "orc_set_wall_density": (None, [vp, ctypes.c_double]),
+        "orc_set_wall_velocity_field": (None, [vp, _dp, _dp, _dp]),
+        "orc_set_wall_velocity_field_scale": (None, [vp, ctypes.c_double]),
+        "orc_set_specular_walls": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint8)]),
+        "orc_set_specular_nodes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint8)]),
         "orc_time_steps": (ctypes.c_double, [vp, ctypes.c_int]),
         "orc_collide_node": (None, [_cfgp, _dp, _dp, _dp]),
         "orc_macroscopic_node": (None, [_cfgp, _dp, _dp]),
@@ -181,6 +190,41 @@ class OracleSolver:
 
     def set_fd_corners(self, on):
         self.L.orc_set_fd_corners(self.h, 1 if on else 0)
+
+    def set_force_correction(self, on):
+        self.L.orc_set_force_correction(self.h, 1 if on else 0)
+
+    def set_outflow_order(self, k):
+        self.L.orc_set_outflow_order(self.h, int(k))
+
+    def set_outflow_density(self, r):
+        self.L.orc_set_outflow_density(self.h, float(r))
+
+    def set_wall_velocity_scale(self, sc):
+        self.L.orc_set_wall_velocity_scale(self.h, float(sc))
+
+    def set_wall_density(self, r):
+        self.L.orc_set_wall_density(self.h, float(r))
+
+    def set_wall_velocity_field(self, ux, uy, uz):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (ux, uy, uz)]
+        n = self.dom[0] * self.dom[1] * self.dom[2]
+        for a in arrs:
+            assert a.size == n
+        self.L.orc_set_wall_velocity_field(self.h, *[a.ctypes.data_as(_dp) for a in arrs])
+
+    def set_wall_velocity_field_scale(self, sc):
+        self.L.orc_set_wall_velocity_field_scale(self.h, float(sc))
+
+    def set_specular_walls(self, normals):
+        a = np.ascontiguousarray(normals, dtype=np.uint8).reshape(-1)
+        if self.L.orc_set_specular_walls(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))):
+            raise ValueError("invalid specular wall specification")
+
+    def set_specular_nodes(self, faces):
+        a = np.ascontiguousarray(faces, dtype=np.uint8).reshape(-1)
+        if self.L.orc_set_specular_nodes(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))):
+            raise ValueError("invalid specular node specification")
 
     def set_force_field(self, fx, fy, fz):
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (fx, fy, fz)]

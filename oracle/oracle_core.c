@@ -1251,15 +1251,25 @@ static void run_step(S* s) {
   const int P = (int)(s->t % 2);
   const int nt = s->nthreads;
   if (s->n_walls && s->streaming == 0) {
-    /* FluidSolver::step runs the corner-density pass first (:583).  Wall
-     * nodes are updated before the fluid nodes: every node reads and writes
-     * only its own slots, so the order changes nothing but what a corner's
-     * finite-difference stencil reads from a fluid neighbour (its incoming
-     * set, before this step). */
+    /* FluidSolver::step: the corner-density pass (:583), then one sweep of
+     * the active list in ascending node order, the order of the reference's
+     * serial execution space, in which its published results were made.
+     * Every node reads and writes only its own slots, so the order matters
+     * only where a wall node reads a NEIGHBOUR's slots (an outlet's upstream
+     * nodes, an outflow donor, a corner's finite-difference stencil): a
+     * neighbour earlier in the order has already been updated and its slots
+     * hold its post-collision set (pair-swapped), a later one has not. */
     if (s->has_corners) corner_density(s, P);
-    for (int64_t k = 0; k < s->n_walls; ++k) wall_update(s, P, s->walls[k]);
-  }
-  if (nt <= 1 || s->n_active < 4096) {
+    int64_t wi = 0;
+    for (int64_t ai = 0; ai <= s->n_active; ++ai) {
+      const int64_t nn = ai < s->n_active ? s->active[ai] : INT64_MAX;
+      while (wi < s->n_walls && s->walls[wi] < nn) wall_update(s, P, s->walls[wi++]);
+      if (ai < s->n_active) {
+        Range r = {s, P, ai, ai + 1};
+        step_range(&r);
+      }
+    }
+  } else if (nt <= 1 || s->n_active < 4096) {
     Range r = {s, P, 0, s->n_active};
     step_range(&r);
   } else {

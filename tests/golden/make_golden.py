@@ -16,6 +16,7 @@ Sources (all in the reference checkout):
   results/B_shear_layer/shear_{d2q9,d3q27}_cm.dat  validation/shear_layer.cpp, L=256 Re=3e4
   results/C_cavity/cavity_re*_{d2q9,d3q27}_cm*.dat validation/cavity_cm.cpp, N=129 u_lid=0.02,
                                                    17 Ghia stations and full centrelines
+  results/A_poiseuille_inlet/poiseuille_*_cm.dat   validation/poiseuille_inlet.cpp, Ly ladder
 """
 import json
 import os
@@ -91,6 +92,12 @@ def main():
                 if os.path.exists(os.path.join(REF, p)):
                     cav[f"re{re_}_{lat}{tail}"] = {"file": p, "header": header(p), "rows": rows(p)}
     data["cavity"] = cav
+
+    pin = {}
+    for lat in ("d2q9", "d3q27"):
+        p = f"results/A_poiseuille_inlet/poiseuille_{lat}_cm.dat"
+        pin[lat] = {"file": p, "header": header(p), "rows": rows(p)}
+    data["poiseuille_inlet"] = pin
 
     with open(os.path.join(REF, "README.md")) as f:
         lines = f.read().splitlines()

@@ -392,3 +392,186 @@ def lerp_at(v, s):
     if i >= len(v) - 1:
         return float(v[-1])
     return float(v[i] + (p - i) * (v[i + 1] - v[i]))
+
+
+def specular_channel(solver_cls, lattice, H, tau, spec, steps, **kw):
+    """validation/specular.cpp:38-70: a Guo-forced BGK channel (second-order
+    equilibrium, shifted), Solid rows at y = 0 and ny - 1; the full channel
+    (ny = 2H + 2) or its lower half (ny = H + 2) with the top row a specular
+    ghost (NrmYp).  Same force in both.  Returns (u_x(y) at x = nx/2,
+    z = nz/2, relative mass drift)."""
+    ny = H + 2 if spec else 2 * H + 2
+    nx, nz = 6, (6 if lattice == abi.D3Q27 else 1)
+    nu = (tau - 0.5) / 3.0
+    umax = 0.02
+    G = 8.0 * nu * umax / (4.0 * H * H)
+    cfg = abi.make_config(lattice, abi.OP_BGK, abi.SHIFTED, abi.F64, (nx, ny, nz),
+                          periodic=(True, False, True), omega=1.0 / tau, force=(G, 0.0, 0.0))
+    s = solver_cls(cfg, **kw)
+    try:
+        fl = np.zeros((nz, ny, nx), dtype=np.uint8)
+        fl[:, 0, :] = abi.SOLID
+        fl[:, ny - 1, :] = abi.SOLID
+        s.set_geometry(fl)
+        if spec:
+            nrm = np.zeros((nz, ny, nx), dtype=np.uint8)
+            nrm[:, ny - 1, :] = cases.NRM_YP
+            s.set_specular_walls(nrm)
+        s.initialize(1.0)
+        m0 = s.total_mass()
+        s.step(steps)
+        drift = (s.total_mass() - m0) / m0
+        ux = s.compute_macroscopic()[1]
+        nxp, nyp, nzp, hx, hy, hz, sx, sy, sz, _ = abi.domain_of(cfg)
+        u = np.asarray(ux).reshape(sz, sy, sx)[hz + nz // 2, hy:hy + ny, hx + nx // 2]
+    finally:
+        s.close()
+    return u, drift
+
+
+def plane_fit(u, H):
+    """validation/specular.cpp:95-121: a quadratic through u(y), y = 1..H;
+    returns (lower root, vertex) = (no-slip plane, symmetry plane)."""
+    y = np.arange(1, H + 1, dtype=np.float64)
+    c2, b, a = np.polyfit(y, np.asarray(u[1:H + 1]), 2)
+    disc = b * b - 4 * c2 * a
+    r1 = (-b + math.sqrt(disc)) / (2 * c2)
+    r2 = (-b - math.sqrt(disc)) / (2 * c2)
+    return min(r1, r2), -b / (2 * c2)
+
+
+def specular_node_2d(solver_cls, lattice, mode, M=16, ny=20, steps=300, tau=0.6, U=0.04,
+                     a=0.02, **kw):
+    """validation/specular_node.cpp:38-96: a solenoidal mode odd/even about
+    x = 0 and x = M, either on the periodic box of width 2M (mode 0) or on
+    [0, M] with on-node specular nodes (mode 1, SpecXm / SpecXp) or specular
+    ghosts (mode 2) at x = 0 and M.  BGK second-order, shifted.  Returns
+    (u_x, u_y interleaved over x = 0..M, y = 0..ny-1; rho)."""
+    half = mode != 0
+    nx = M + 1 if half else 2 * M
+    cfg = abi.make_config(lattice, abi.OP_BGK, abi.SHIFTED, abi.F64, (nx, ny, 1),
+                          periodic=(not half, True, True), omega=1.0 / tau)
+    s = solver_cls(cfg, **kw)
+    try:
+        if mode == 1:
+            f = np.zeros((1, ny, nx), dtype=np.uint8)
+            f[:, :, 0] = cases.SPEC_XM
+            f[:, :, nx - 1] = cases.SPEC_XP
+            s.set_specular_nodes(f)
+        elif mode == 2:
+            f = np.zeros((1, ny, nx), dtype=np.uint8)
+            f[:, :, 0] = cases.NRM_XM
+            f[:, :, nx - 1] = cases.NRM_XP
+            s.set_specular_walls(f)
+        px, py, pz, d = cases.padded_coords(cfg)
+        x = (px - d[3]).astype(np.float64)
+        y = (py - d[4]).astype(np.float64)
+        ux = -U * np.sin(math.pi * x / M) * np.sin(2 * math.pi * y / ny)
+        uy = -U * (ny / (2.0 * M)) * np.cos(math.pi * x / M) * np.cos(2 * math.pi * y / ny)
+        rr = 1.0 + a * np.cos(math.pi * x / M) * np.cos(2 * math.pi * y / ny)
+        s.initialize_field(rr, ux, uy, np.zeros_like(ux))
+        s.step(steps)
+        r, u, v, _ = (interior2d(cfg, q) for q in s.compute_macroscopic())
+    finally:
+        s.close()
+    uv = np.stack([u[:, :M + 1], v[:, :M + 1]], axis=-1).reshape(-1)
+    return uv, r[:, :M + 1].reshape(-1)
+
+
+def specular_node_3d(solver_cls, quarter, M=16, ny=20, P=12, steps=300, tau=0.6, U=0.04,
+                     a=0.02, **kw):
+    """validation/specular_node.cpp:98-160, D3Q27: the full periodic box
+    (2M x ny x 2P) or its quarter [0, M] x [0, P] with on-node specular faces
+    at x = 0, M and z = 0, P (edges carry two faces).  Returns (u, v, w over
+    the quarter; rho)."""
+    nx = M + 1 if quarter else 2 * M
+    nz = P + 1 if quarter else 2 * P
+    cfg = abi.make_config(abi.D3Q27, abi.OP_BGK, abi.SHIFTED, abi.F64, (nx, ny, nz),
+                          periodic=(not quarter, True, not quarter), omega=1.0 / tau)
+    s = solver_cls(cfg, **kw)
+    try:
+        if quarter:
+            z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+            f = np.zeros(x.shape, dtype=np.uint8)
+            f |= np.where(x == 0, cases.SPEC_XM, 0).astype(np.uint8)
+            f |= np.where(x == nx - 1, cases.SPEC_XP, 0).astype(np.uint8)
+            f |= np.where(z == 0, cases.SPEC_ZM, 0).astype(np.uint8)
+            f |= np.where(z == nz - 1, cases.SPEC_ZP, 0).astype(np.uint8)
+            s.set_specular_nodes(f)
+        px, py, pz, d = cases.padded_coords(cfg)
+        x = (px - d[3]).astype(np.float64)
+        y = (py - d[4]).astype(np.float64)
+        z = (pz - d[5]).astype(np.float64)
+        sc = U / max(M, P)
+        A, B, C = sc * M, -sc * ny * 0.5, sc * P
+        sx, cx = np.sin(math.pi * x / M), np.cos(math.pi * x / M)
+        sz, cz = np.sin(math.pi * z / P), np.cos(math.pi * z / P)
+        sy, cy = np.sin(2 * math.pi * y / ny), np.cos(2 * math.pi * y / ny)
+        s.initialize_field(1.0 + a * cx * cy * cz, A * sx * cy * cz, B * cx * sy * cz,
+                           C * cx * cy * sz)
+        s.step(steps)
+        mac = s.compute_macroscopic()
+        nxp, nyp, nzp, hx, hy, hz, sxp, syp, szp, _ = abi.domain_of(cfg)
+        cut = [np.asarray(q).reshape(szp, syp, sxp)[hz:hz + P + 1, hy:hy + ny, hx:hx + M + 1]
+               for q in mac]
+    finally:
+        s.close()
+    return np.stack(cut[1:], axis=-1).reshape(-1), cut[0].reshape(-1)
+
+
+def poiseuille_inlet_run(solver_cls, lattice, Ly, oo=1, U0=0.005, Re=10.0, Lx=21, ctol=1e-10,
+                         **kw):
+    """validation/poiseuille_inlet.cpp:118-205 with its defaults: a channel of
+    Lx x Ly (x, y walls, z periodic, Lz = 1), regularised walls at y = 0 and
+    Ly - 1, a parabolic velocity inlet at x = 0 (NrmXm, u = 4 U0 s (1 - s)),
+    a constant back-pressure outlet at x = Lx - 1 (NrmOutXp, rho = 1) with
+    tangential order `oo`; CM shifted, nu = U0 Ly / Re, from rest; the
+    whole-field u_x change over probe = max(500, tau_d / 200) steps below ctol
+    after at least tau_d / 2 steps, tau_d = Re Ly / U0.  Returns (relative L2
+    error of u_x(x = Lx/2) against the parabola, steps, residual)."""
+    nu = U0 * Ly / Re
+    cfg = abi.make_config(lattice, abi.OP_CM, abi.SHIFTED, abi.F64, (Lx, Ly, 1),
+                          periodic=(False, False, True), omega=1.0 / (3.0 * nu + 0.5))
+    z, y, x = np.meshgrid(np.arange(1), np.arange(Ly), np.arange(Lx), indexing="ij")
+    nrm = np.zeros(x.shape, dtype=np.uint8)
+    vel = np.zeros(x.shape + (4,))
+    vel[..., 3] = 1.0  # WallSpec::rho defaults to 1 (FluidSolver.hpp:100-102)
+    outlet = (x == Lx - 1) & (y > 0) & (y < Ly - 1)
+    nrm[x == 0] = cases.NRM_XM
+    nrm[outlet] = cases.NRM_OUT_XP
+    nrm[y == 0] = cases.NRM_YM
+    nrm[y == Ly - 1] = cases.NRM_YP
+    yy = y / (Ly - 1.0)
+    vel[..., 0] = np.where(x == 0, 4.0 * U0 * yy * (1.0 - yy), 0.0)
+    states = {}
+    tag = np.zeros(x.shape, dtype=np.uint16)
+    for k in zip(*np.nonzero(nrm)):
+        key = tuple(vel[k])
+        tag[k] = states.setdefault(key, len(states))
+    table = np.array(sorted(states, key=states.get))
+    s = solver_cls(cfg, **kw)
+    try:
+        s.set_regularized_walls(nrm.reshape(-1), tag.reshape(-1), table)
+        s.set_outflow_order(oo)
+        s.initialize(1.0)
+        tau_d = Re * Ly / U0
+        probe = max(500, int(tau_d / 200.0))
+        tmin = int(0.5 * tau_d)
+        cap = int(30.0 * tau_d)
+        prev = np.zeros((Ly, Lx))
+        taken, res = 0, float("nan")
+        for _ in range(0, cap, probe):
+            s.step(probe)
+            taken += probe
+            u = interior2d(cfg, s.compute_macroscopic()[1])
+            assert np.all(np.isfinite(u))
+            res = math.sqrt(float(np.sum((u - prev) ** 2)) / max(float(np.sum(u * u)), 1e-300))
+            prev = u
+            if taken >= tmin and res < ctol:
+                break
+        u = interior2d(cfg, s.compute_macroscopic()[1])[:, Lx // 2]
+    finally:
+        s.close()
+    s_ = np.arange(Ly) / (Ly - 1.0)
+    ana = 4.0 * U0 * s_ * (1.0 - s_)
+    return math.sqrt(float(np.sum((u - ana) ** 2) / np.sum(ana ** 2))), taken, res

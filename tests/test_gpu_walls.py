@@ -200,3 +200,234 @@ def test_cavity_against_reference_results(name, lat, Re, restol):
         ev = np.abs(vc - rows[:, 2]).max()
         print(f"{name}: full centrelines max |du| {eu:.2e} |dv| {ev:.2e}", flush=True)
         assert eu <= 5.1e-9 and ev <= 5.1e-9, (eu, ev)
+
+
+# ---------------------------------------------------------------------------
+# Outflow codes, specular cells, wall-state controls: device against oracle
+# ---------------------------------------------------------------------------
+from golden_cases import (plane_fit, poiseuille_inlet_run, specular_channel,  # noqa: E402
+                          specular_node_2d, specular_node_3d)
+
+
+def _outflow_walls(shape, code, faces=("x",)):
+    """Walls at y = 0 and ny-1 (at rest), a parabolic inlet at x = 0 and the
+    outflow code on the x = nx-1 column (and, for faces containing 'ybottom',
+    on the floor instead of a wall)."""
+    nx, ny, nz = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    nrm = np.zeros(x.shape, dtype=np.uint8)
+    nrm[x == 0] = cases.NRM_XM
+    nrm[x == nx - 1] = code
+    nrm[y == ny - 1] = cases.NRM_YP
+    nrm[y == 0] = code if "ybottom" in faces else cases.NRM_YM
+    tag = np.zeros(x.shape, dtype=np.uint16)
+    tag[(x == 0) & (y > 0) & (y < ny - 1)] = 1
+    tag[nrm == code] = 2
+    table = np.array([[0.0, 0.0, 0.0, 1.0], [0.03, 0.0, 0.0, 1.0], [0.0, 0.0, 0.0, 1.01]])
+    return nrm.reshape(-1), tag.reshape(-1), table
+
+
+OUTFLOW = [(cases.NRM_OUT_XP, 1), (cases.NRM_OUT_XP, 2), (cases.NRM_OUT_EQ, 1),
+           (cases.NRM_OUT_FREE, 1)]
+
+
+@pytest.mark.parametrize("dtype", [abi.F64, abi.F32])
+@pytest.mark.parametrize("op", [abi.OP_BGK, abi.OP_CM])
+@pytest.mark.parametrize("lat,nz", [(abi.D2Q9, 1), (abi.D3Q27, 3)])
+@pytest.mark.parametrize("code,order", OUTFLOW)
+def test_outflow_codes_match_oracle(code, order, lat, nz, op, dtype):
+    """The outlets read their upstream node after its update in the same
+    step (the reference's node order), on the device through the wall pass
+    that runs after the step."""
+    shape = (14, 9, nz)
+    cfg = abi.make_config(lat, op, abi.SHIFTED, dtype, shape, periodic=(False, False, True),
+                          omega=1.3, omega_bulk=1.1 if op == abi.OP_CM else -1.0)
+    s, o = _pair(cfg, _outflow_walls(shape, code))
+    with s:
+        for x in (s, o):
+            x.set_outflow_order(order)
+            x.set_outflow_density(0.99)
+        _compare(s, o, dtype)
+
+
+@pytest.mark.parametrize("dtype", [abi.F64, abi.F32])
+def test_outflow_donor_later_in_the_order(dtype):
+    """Free outlets on the floor take their donor from y = 1, a LATER node:
+    it is read before its update (the copy taken before the step)."""
+    shape = (12, 10, 1)
+    cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, dtype, shape,
+                          periodic=(False, False, True), omega=1.2)
+    s, o = _pair(cfg, _outflow_walls(shape, cases.NRM_OUT_FREE, faces=("x", "ybottom")))
+    with s:
+        _compare(s, o, dtype)
+
+
+@pytest.mark.parametrize("lat,nz", [(abi.D2Q9, 1), (abi.D3Q27, 4)])
+def test_corner_stencil_reads_fluid_on_both_sides(lat, nz):
+    """A lid segment ending in a corner node inside the box: its stencil
+    reads fluid nodes earlier in the order (below it, already updated) and
+    later (beside it, not yet)."""
+    N = 14
+    cfg = abi.make_config(lat, abi.OP_CM, abi.SHIFTED, abi.F64, (N, N, nz),
+                          periodic=(False, False, True), omega=1.25)
+    nrm, tag, table = cases.cavity_walls(N, 0.05, nz)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(N), np.arange(N), indexing="ij")
+    nrm = nrm.reshape(x.shape).copy()
+    tag = tag.reshape(x.shape).copy()
+    nrm[(y == N - 1) & (x > 6)] = 0          # the lid stops at x = 6 ...
+    nrm[(y == N - 1) & (x == 6)] = cases.NRM_CORNER  # ... in a corner node
+    nrm[(y == N - 1) & (x == N - 1)] = cases.NRM_XP
+    s, o = _pair(cfg, (nrm.reshape(-1), tag.reshape(-1), table))
+    with s:
+        _compare(s, o, abi.F64)
+
+
+@pytest.mark.parametrize("dtype", [abi.F64, abi.F32])
+@pytest.mark.parametrize("lat", [abi.D2Q9, abi.D3Q27])
+def test_specular_cells_match_oracle(lat, dtype):
+    """Ghost specular rows and on-node specular faces (with an edge) beside
+    regularised walls and a Solid block, forced."""
+    nx, ny, nz = 12, 11, (1 if lat == abi.D2Q9 else 5)
+    cfg = abi.make_config(lat, abi.OP_CM, abi.SHIFTED, dtype, (nx, ny, nz),
+                          periodic=(True, False, lat == abi.D2Q9), omega=1.3,
+                          force=(1e-5, 0.0, 0.0))
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    fl = np.zeros(x.shape, dtype=np.uint8)
+    fl[:, 0, :] = abi.SOLID
+    fl[:, 3:5, 4:6] = abi.SOLID
+    spec = np.zeros(x.shape, dtype=np.uint8)
+    spec[:, ny - 1, :] = cases.NRM_YP
+    faces = np.zeros(x.shape, dtype=np.uint8)
+    faces[:, ny - 2, :] = cases.SPEC_YP
+    if nz > 1:
+        faces[0, 1:ny - 1, :] |= cases.SPEC_ZM
+        faces[nz - 1, 1:ny - 1, :] |= cases.SPEC_ZP
+    faces[:, ny - 2, 0:3] = 0
+    nrm = np.zeros(x.shape, dtype=np.uint8)
+    nrm[:, ny - 2, 0:3] = cases.NRM_YP
+    o = OracleSolver(cfg)
+    s = FluidSolver(cfg)
+    for q in (s, o):
+        q.set_geometry(fl)
+        q.set_specular_walls(spec)
+        q.set_specular_nodes(faces)
+        q.set_regularized_walls(nrm.reshape(-1), np.zeros(nrm.size, dtype=np.uint16),
+                                np.array([[0.01, 0.0, 0.0, 1.0]]))
+    field = _field(cfg, o.np)
+    o.initialize_field(*field)
+    s.initialize_field(*field)
+    with s:
+        _compare(s, o, dtype)
+
+
+def test_wall_state_controls_match_oracle():
+    """Velocity scale, wall density, per-node velocity field and its scale,
+    and the force correction switched off, on a forced channel with an
+    outlet."""
+    shape = (14, 9, 1)
+    cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, abi.F64, shape,
+                          periodic=(False, False, True), omega=1.3, force=(2e-6, 0.0, 0.0))
+    s, o = _pair(cfg, _outflow_walls(shape, cases.NRM_OUT_XP))
+    rng = np.random.default_rng(2)
+    u = 0.01 * rng.standard_normal(shape[0] * shape[1])
+    with s:
+        for q in (s, o):
+            q.set_wall_velocity_scale(0.7)
+            q.set_wall_density(1.005)
+            q.set_force_correction(False)
+        _compare(s, o, abi.F64, chunks=(1, 9))
+        for q in (s, o):
+            q.set_wall_velocity_field(u, 0.3 * u, 0 * u)
+            q.set_wall_velocity_field_scale(0.5)
+            q.set_force_correction(True)
+        _compare(s, o, abi.F64, chunks=(1, 9))
+
+
+def test_stored_macroscopic_at_boundary_nodes():
+    """step(store_macroscopic) writes the state the step collided: at a
+    velocity wall that is the imposed velocity and the closure density, at an
+    on-node specular node the mirrored set's moments -- what
+    compute_macroscopic reports just before the step."""
+    nx, ny = 12, 9
+    cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, abi.F64, (nx, ny, 1),
+                          periodic=(True, False, True), omega=1.3)
+    y, x = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    nrm = np.where(y == 0, cases.NRM_YM, 0).astype(np.uint8)
+    faces = np.where(y == ny - 1, cases.SPEC_YP, 0).astype(np.uint8)
+    with FluidSolver(cfg) as s:
+        s.set_regularized_walls(nrm.reshape(-1), np.zeros(nrm.size, dtype=np.uint16),
+                                np.array([[0.02, 0.0, 0.0, 1.0]]))
+        s.set_specular_nodes(faces)
+        s.initialize_field(*_field(cfg, s.np))
+        s.step(7)
+        before = s.compute_macroscopic()
+        s.step(1, store_macroscopic=True)
+        stored = s.macroscopic_stored()
+        for a, b in zip(before, stored):
+            np.testing.assert_allclose(b, a, rtol=1e-13, atol=1e-15)
+
+
+def test_boundary_argument_checks():
+    cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, abi.F64, (8, 8, 1),
+                          periodic=(False, False, True))
+    with FluidSolver(cfg) as s:
+        with pytest.raises(RuntimeError):
+            s.set_wall_velocity_scale(2.0)      # no walls yet
+        bad = np.zeros(64, dtype=np.uint8)
+        bad[9] = cases.NRM_CORNER
+        with pytest.raises(RuntimeError):
+            s.set_specular_walls(bad)
+        both = np.zeros(64, dtype=np.uint8)
+        both[9] = cases.SPEC_YP | cases.SPEC_YM
+        with pytest.raises(RuntimeError):
+            s.set_specular_nodes(both)
+        out = np.zeros(64, dtype=np.uint8)
+        out[8 * 3 + 1] = cases.NRM_OUT_XP       # x = 1: no second upstream node
+        with pytest.raises(RuntimeError):
+            s.set_regularized_walls(out, np.zeros(64, dtype=np.uint16),
+                                    np.array([[0.0, 0.0, 0.0, 1.0]]))
+
+
+@pytest.mark.parametrize("lat,Ly", [(abi.D2Q9, 41), (abi.D2Q9, 81), (abi.D2Q9, 161),
+                                    (abi.D3Q27, 41), (abi.D3Q27, 81)])
+def test_inlet_poiseuille_against_reference_results(lat, Ly):
+    """results/A_poiseuille_inlet/poiseuille_<lat>_cm.dat: error to its 9
+    printed digits and the step count of every row run here."""
+    name = "d2q9" if lat == abi.D2Q9 else "d3q27"
+    row = next(r for r in GOLD["poiseuille_inlet"][name]["rows"] if int(r[0]) == Ly)
+    err, steps, _ = poiseuille_inlet_run(FluidSolver, lat, Ly)
+    print(f"inlet Poiseuille {name} Ly={Ly}: err {err:.8e} (file {row[5]:.8e}), "
+          f"steps {steps} (file {int(row[2])})", flush=True)
+    assert steps == int(row[2])
+    assert abs(err - row[5]) <= 6e-9 * row[5]
+
+
+def test_inlet_poiseuille_outflow_order_two_on_device():
+    err, _, _ = poiseuille_inlet_run(FluidSolver, abi.D2Q9, 41, oo=2)
+    assert abs(err - 1.388e-3) <= 5e-7, err
+
+
+@pytest.mark.parametrize("lat,tau", [(abi.D2Q9, 0.8), (abi.D2Q9, 1.2), (abi.D3Q27, 0.8)])
+def test_specular_ghost_identity_on_device(lat, tau):
+    """validation/specular.cpp at its own settings (H = 16, 40000 steps)."""
+    H = 16
+    uf, _ = specular_channel(FluidSolver, lat, H, tau, False, 40000)
+    uh, drift = specular_channel(FluidSolver, lat, H, tau, True, 40000)
+    rel = np.abs(uh[1:H + 1] - uf[1:H + 1]).max() / np.abs(uf[1:H + 1]).max()
+    assert rel < 1e-12, rel
+    lo, vertex = plane_fit(uh, H)
+    assert abs(vertex - (H + 0.5)) < 0.02 and abs(lo - 0.5) < 0.02, (lo, vertex)
+    assert abs(drift) < 1e-13
+
+
+@pytest.mark.parametrize("lat", [abi.D2Q9, abi.D3Q27])
+def test_specular_node_identity_on_device(lat):
+    a, ra = specular_node_2d(FluidSolver, lat, 0)
+    b, rb = specular_node_2d(FluidSolver, lat, 1)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    assert np.abs(ra - rb).max() <= 1e-12
+    if lat == abi.D3Q27:
+        a, ra = specular_node_3d(FluidSolver, False)
+        b, rb = specular_node_3d(FluidSolver, True)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+        assert np.abs(ra - rb).max() <= 1e-12

@@ -6,9 +6,11 @@
  * reference's configure-time typedef (src/core/Types.hpp:16-20).  Every routine
  * restates the reference routine cited beside it, in the reference's operation
  * order, so an FP64 run reproduces the reference's arithmetic term for term.
- * Node order is ascending, which is Kokkos Serial's order; the esoteric-pull
- * step is order-independent (src/memory/EsotericPull.hpp:19-21), the seeding
- * scatter is not at halo layers and ascending order is what it leaves.
+ * Node order is ascending, which is Kokkos Serial's order, the order the
+ * reference's published results were made in.  The esoteric-pull step is
+ * order-independent (src/memory/EsotericPull.hpp:19-21) except where a wall
+ * node reads a neighbour's slots (outlets, corner stencils), and the seeding
+ * scatter is not at halo layers; ascending order is what both leave.
  */
 #include <math.h>
 #include <pthread.h>

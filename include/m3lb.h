@@ -296,6 +296,14 @@ int m3lb_time_steps(m3lb_solver* s, int32_t nsteps, double* ms);
  * counterpart in the reference. */
 int m3lb_set_tile_order(m3lb_solver* s, int32_t run);
 
+/* How the step kernel accesses the slots a node owns at its own index:
+ * nontemporal (STREAM) or plain (PLAIN).  AUTO, the default, takes PLAIN when
+ * the population array fits the 256 MB last-level cache (there it streams
+ * about 25% faster) and STREAM above.  A tuning knob only: results are
+ * bit-identical for every value.  No counterpart in the reference. */
+enum { M3LB_CACHE_AUTO = 0, M3LB_CACHE_STREAM = 1, M3LB_CACHE_PLAIN = 2 };
+int m3lb_set_cache_policy(m3lb_solver* s, int32_t policy);
+
 /* The block -> tile map the step kernel uses for `grid` blocks and a tile
  * order `run` (host-side copy of the device function): out[b] = tile of block
  * b.  A permutation of [0, grid) for every value; exposed so that this can be

@@ -278,3 +278,5 @@ def request_config(scfg, req):
                     omega_bulk=req.omega_bulk,
                     force=tuple(req.force) if scfg.forcing == FORCE_GUO else None)
     return c
+
+CACHE_AUTO, CACHE_STREAM, CACHE_PLAIN = 0, 1, 2

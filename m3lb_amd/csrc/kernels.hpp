@@ -75,6 +75,7 @@ struct StepArgs {
   int chunks;      // x-chunks per row (1 unless nx > 256)
   int rows;        // ny * nz
   int remap;       // tile order, see tile_of()
+  int cached;      // 1: plain accesses everywhere (a cache-resident working set)
   double omega, omega_m, omega_b;
   double F[3];
   void* macro[4];  // rho, ux, uy, uz (compute type), or null
@@ -144,14 +145,17 @@ template <class T>
 __device__ __forceinline__ const T* qbase(const T* A, uint64_t stride, int q) {
   return A + uint64_t(q) * stride;
 }
-// Population accesses.  M3LB_NT selects the cache policy of the slots a node
-// owns at its own index ("local", aligned with the wavefront) and of those at
-// n + c_i ("remote", shifted by one element along x so that neighbouring
-// wavefronts share their boundary cache lines): 0 plain everywhere, 1
-// nontemporal on local slots only, 2 nontemporal everywhere.
-// Measured at 512^3 on MI355X (D3Q27, every operator and dtype): 1 is the
-// fastest; nontemporal remote accesses lose up to 15% because a boundary line
-// then reaches HBM once per wavefront that touches it.
+// Population accesses.  The slots a node owns at its own index ("local",
+// aligned with the wavefront) are read and written nontemporally when the
+// array is far larger than the caches (the step kernels' NTL template flag,
+// chosen at run time from the working set); the slots at n + c_i ("remote",
+// shifted by one element along x so that neighbouring wavefronts share their
+// boundary cache lines) always use plain accesses: nontemporal remote
+// accesses lose up to 15% because a boundary line then reaches HBM once per
+// wavefront that touches it.  A working set that fits the 256 MB of last-level
+// cache streams about 25% faster with plain accesses everywhere (measured
+// D3Q27 CM FP32: 64^3..128^3 at 28-30 against 24 GLUPS).  M3LB_NT sets the
+// default the other kernels use: 0 plain everywhere, 1 nontemporal local.
 #ifndef M3LB_NT
 #define M3LB_NT 1
 #endif
@@ -182,24 +186,27 @@ __device__ __forceinline__ void st(T* base, unsigned node, T v) {
   else *p = __builtin_bit_cast(W, v);
 }
 constexpr bool kNtLocal = M3LB_NT >= 1;
-constexpr bool kNtRemote = M3LB_NT >= 2;
+constexpr bool kNtRemote = false;
+// Population arrays up to this size run the step kernels with plain
+// accesses (the MI355X last-level cache is 256 MB).
+constexpr uint64_t kCacheResidentBytes = 256ull << 20;
 
 // load_rest / load_pair of EsotericPull::Access<P> (EsotericPull.hpp:213-223)
-template <class K, int P>
+template <class K, int P, bool NTL = kNtLocal>
 __device__ __forceinline__ void load_node(const typename K::T* __restrict__ A, uint64_t stride,
                                           unsigned n, const Offs& o, const float* inv,
                                           typename K::R* f) {
   using C = typename K::C;
   constexpr int Q = K::Q;
-  f[0] = C::dec(ld<kNtLocal>(qbase(A, stride, 0), n), inv[0]);
+  f[0] = C::dec(ld<NTL>(qbase(A, stride, 0), n), inv[0]);
   sfor<(Q - 1) / 2>([&](auto Pr) {
     constexpr int i = 2 * Pr + 1;
     const unsigned nb = unsigned(int(n) + nb_off<Q, i>(o));
     if constexpr (P == 0) {
-      f[i] = C::dec(ld<kNtLocal>(qbase(A, stride, i), n), inv[i]);
+      f[i] = C::dec(ld<NTL>(qbase(A, stride, i), n), inv[i]);
       f[i + 1] = C::dec(ld<kNtRemote>(qbase(A, stride, i + 1), nb), inv[i + 1]);
     } else {
-      f[i] = C::dec(ld<kNtLocal>(qbase(A, stride, i + 1), n), inv[i + 1]);
+      f[i] = C::dec(ld<NTL>(qbase(A, stride, i + 1), n), inv[i + 1]);
       f[i + 1] = C::dec(ld<kNtRemote>(qbase(A, stride, i), nb), inv[i]);
     }
   });
@@ -207,7 +214,7 @@ __device__ __forceinline__ void load_node(const typename K::T* __restrict__ A, u
 
 // store_rest / store_pair (EsotericPull.hpp:218-224): writes crossed into the
 // very slots the load read.
-template <class K, int P>
+template <class K, int P, bool NTL = kNtLocal>
 __device__ __forceinline__ void store_node(typename K::T* __restrict__ A, uint64_t stride,
                                            unsigned n, const Offs& o, const float* s,
                                            const typename K::R* f) {
@@ -215,16 +222,16 @@ __device__ __forceinline__ void store_node(typename K::T* __restrict__ A, uint64
   using T = typename K::T;
   constexpr int Q = K::Q;
   auto wb = [&](int q) { return const_cast<T*>(qbase<T>(A, stride, q)); };
-  st<kNtLocal>(wb(0), n, C::enc(f[0], s[0]));
+  st<NTL>(wb(0), n, C::enc(f[0], s[0]));
   sfor<(Q - 1) / 2>([&](auto Pr) {
     constexpr int i = 2 * Pr + 1;
     const unsigned nb = unsigned(int(n) + nb_off<Q, i>(o));
     if constexpr (P == 0) {
       st<kNtRemote>(wb(i + 1), nb, C::enc(f[i], s[i + 1]));
-      st<kNtLocal>(wb(i), n, C::enc(f[i + 1], s[i]));
+      st<NTL>(wb(i), n, C::enc(f[i + 1], s[i]));
     } else {
       st<kNtRemote>(wb(i), nb, C::enc(f[i], s[i]));
-      st<kNtLocal>(wb(i + 1), n, C::enc(f[i + 1], s[i + 1]));
+      st<NTL>(wb(i + 1), n, C::enc(f[i + 1], s[i + 1]));
     }
   });
 }
@@ -271,7 +278,7 @@ __device__ __forceinline__ void tile_node(unsigned tile, int chunks, int bx_log2
 // bounce-back, :759-761); a fluid node loads its incoming set, collides and
 // stores it back crossed.  Shared by the single-box and the batched kernels,
 // so a box computes bit-identical results in either.
-template <class K, int P, bool GEOM>
+template <class K, int P, bool GEOM, bool NTL>
 __device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint64_t stride,
                                             const uint8_t* flags, const Dom& d, int x, int y,
                                             int z, const Params<typename K::R>& p0,
@@ -286,9 +293,9 @@ __device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint6
   force_at<K>(p, ff, stride, n);
   const Offs o = offsets(d, x, y, z);
   R f[K::Q];
-  load_node<K, P>(A, stride, n, o, inv_scale, f);
+  load_node<K, P, NTL>(A, stride, n, o, inv_scale, f);
   const Macro<R> m = collide_fused<K>(f, p);
-  store_node<K, P>(A, stride, n, o, scale, f);
+  store_node<K, P, NTL>(A, stride, n, o, scale, f);
   if (macro && macro[0]) {
     static_cast<R*>(macro[0])[n] = density<K>(m);
     static_cast<R*>(macro[1])[n] = m.ux;
@@ -298,14 +305,16 @@ __device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint6
 }
 
 // run_step<P, StoreMacro> over one box: one thread per node, tiles of 256.
-template <class K, int P, bool GEOM>
+// NTL: nontemporal accesses to the local slots (working sets far beyond the
+// caches).
+template <class K, int P, bool GEOM, bool NTL>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   int row, x;
   tile_node(tile_of(blockIdx.x, gridDim.x, a.remap), a.chunks, a.bx_log2, row, x);
   const Dom& d = a.d;
   if (row >= a.rows || x >= d.nx) return;
   const Params<typename K::R> p = params_of<K>(a);
-  node_update<K, P, GEOM>(static_cast<typename K::T*>(a.f), a.stride, a.flags, d, x,
+  node_update<K, P, GEOM, NTL>(static_cast<typename K::T*>(a.f), a.stride, a.flags, d, x,
                           row % d.ny, row / d.ny, p, a.scale, a.inv_scale, a.macro, a.ff);
 }
 
@@ -344,10 +353,11 @@ struct BatchArgs {
   const unsigned* hdr;  // {boxes, total tiles, tile_start[0..boxes]}
   int j;                // step within the chunk
   int run;              // tile order, see tile_of()
+  int cached;           // 1: plain accesses everywhere (resident set in cache)
 };
 
 // One tile of box b at step j of the chunk.
-template <class K>
+template <class K, bool NTL>
 __device__ __forceinline__ void batched_tile(const BoxDesc& b, unsigned local, int j) {
   using R = typename K::R;
   using T = typename K::T;
@@ -375,9 +385,9 @@ __device__ __forceinline__ void batched_tile(const BoxDesc& b, unsigned local, i
   // Flags are always checked: a second (flag-free) variant of the node body
   // next to the two parities would take the kernel from 67 to 113 VGPRs.
   if (((b.parity0 + j) & 1) == 0)
-    node_update<K, 0, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr, nullptr);
+    node_update<K, 0, true, NTL>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr, nullptr);
   else
-    node_update<K, 1, true>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr, nullptr);
+    node_update<K, 1, true, NTL>(A, b.stride, b.flags, b.d, x, y, z, p, b.scale, b.inv_scale, nullptr, nullptr);
 }
 
 // Tile order of the batched kernel: run >= -1 as tile_of (virtual tile
@@ -389,7 +399,7 @@ constexpr int kRunBlockRanges = -2;
 
 // RANGES: the kRunBlockRanges order (a separate instance, so that neither
 // order's loop raises the other's register count).
-template <class K, bool RANGES>
+template <class K, bool RANGES, bool NTL>
 __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
   const unsigned nbox = a.hdr[0], total = a.hdr[1];
   const unsigned* start = a.hdr + 2;  // start[nbox] == total
@@ -409,7 +419,7 @@ __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
       const BoxDesc& b = a.desc[lo];
       const unsigned stop = min(end, start[lo + 1]);
       if (a.j < b.steps)
-        for (; t < stop; ++t) batched_tile<K>(b, t - start[lo], a.j);
+        for (; t < stop; ++t) batched_tile<K, NTL>(b, t - start[lo], a.j);
       t = stop;
     }
     return;
@@ -424,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
     }
     const BoxDesc& b = a.desc[lo];
     if (a.j >= b.steps) continue;
-    batched_tile<K>(b, t - start[lo], a.j);
+    batched_tile<K, NTL>(b, t - start[lo], a.j);
   }
 }
 

@@ -39,13 +39,18 @@ inline unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBl
 template <class K>
 struct Launch {
   static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::C::scaled && !K::FIELD;
+  template <int P, bool G>
+  static void step_p(const StepArgs& a, unsigned grid, hipStream_t s) {
+    if (a.cached) k_step<K, P, G, false><<<grid, kBlock, 0, s>>>(a);
+    else k_step<K, P, G, true><<<grid, kBlock, 0, s>>>(a);
+  }
   static void step(const StepArgs& a, int parity, bool geom, unsigned grid, hipStream_t s) {
     if (parity == 0) {
-      if (geom) k_step<K, 0, true><<<grid, kBlock, 0, s>>>(a);
-      else k_step<K, 0, false><<<grid, kBlock, 0, s>>>(a);
+      if (geom) step_p<0, true>(a, grid, s);
+      else step_p<0, false>(a, grid, s);
     } else {
-      if (geom) k_step<K, 1, true><<<grid, kBlock, 0, s>>>(a);
-      else k_step<K, 1, false><<<grid, kBlock, 0, s>>>(a);
+      if (geom) step_p<1, true>(a, grid, s);
+      else step_p<1, false>(a, grid, s);
     }
   }
   static void seed(const SeedArgs& a, hipStream_t s) {
@@ -105,8 +110,13 @@ struct Launch {
   }
   static void batched(const BatchArgs& a, unsigned grid, hipStream_t s) {
     if constexpr (kBatched) {
-      if (a.run == kRunBlockRanges) k_step_batched<K, true><<<grid, kBlock, 0, s>>>(a);
-      else k_step_batched<K, false><<<grid, kBlock, 0, s>>>(a);
+      if (a.run == kRunBlockRanges) {
+        if (a.cached) k_step_batched<K, true, false><<<grid, kBlock, 0, s>>>(a);
+        else k_step_batched<K, true, true><<<grid, kBlock, 0, s>>>(a);
+      } else {
+        if (a.cached) k_step_batched<K, false, false><<<grid, kBlock, 0, s>>>(a);
+        else k_step_batched<K, false, true><<<grid, kBlock, 0, s>>>(a);
+      }
     }
   }
   static const KernelSet* table() {

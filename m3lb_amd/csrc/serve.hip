@@ -336,6 +336,11 @@ m3lb::BatchArgs batch_args(const m3lb_server* s, int buf, int j) {
   a.hdr = s->hdr_d[buf];
   a.j = j;
   a.run = s->cfg.tile_run;
+  // a node cap whose populations fit the last-level cache keeps the whole
+  // resident set there: plain accesses then stream faster
+  a.cached = s->cfg.max_resident_nodes > 0 &&
+             s->cfg.max_resident_nodes * uint64_t(s->Q) * uint64_t(s->ks->elem_bytes) <=
+                 m3lb::kCacheResidentBytes;
   return a;
 }
 

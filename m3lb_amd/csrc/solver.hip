@@ -138,6 +138,7 @@ struct m3lb_solver {
   float inv_scale[27];
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int remap = 64;  // step-kernel tile order (kernels.hpp tile_of)
+  int cache_policy = M3LB_CACHE_AUTO;  // step-kernel accesses to the local slots
   void* ff_d = nullptr;  // M3LB_FORCE_FIELD: fx, fy, fz planes of `stride` (compute type)
   Boundaries b;  // regularised walls and specular cells
 };
@@ -169,6 +170,9 @@ m3lb::StepArgs step_args(const m3lb_solver* s) {
   a.chunks = (nx + m3lb::kBlock - 1) / m3lb::kBlock;
   a.rows = s->d.ny * s->d.nz;
   a.remap = s->remap;
+  const uint64_t bytes = s->stride * uint64_t(s->Q) * uint64_t(s->ks->elem_bytes);
+  a.cached = s->cache_policy == M3LB_CACHE_PLAIN ||
+             (s->cache_policy == M3LB_CACHE_AUTO && bytes <= m3lb::kCacheResidentBytes);
   a.omega = s->cfg.omega;
   a.omega_m = s->cfg.omega_minus;
   a.omega_b = s->cfg.omega_bulk;
@@ -1426,6 +1430,14 @@ int m3lb_set_scales(m3lb_solver* s, const double* sc) {
 int m3lb_set_tile_order(m3lb_solver* s, int32_t run) {
   if (!s) return fail(M3LB_EINVAL, "null handle");
   s->remap = run;
+  return M3LB_OK;
+}
+
+int m3lb_set_cache_policy(m3lb_solver* s, int32_t policy) {
+  if (!s) return fail(M3LB_EINVAL, "null handle");
+  if (policy < M3LB_CACHE_AUTO || policy > M3LB_CACHE_PLAIN)
+    return fail(M3LB_EINVAL, "cache policy is AUTO (0), STREAM (1) or PLAIN (2)");
+  s->cache_policy = policy;
   return M3LB_OK;
 }
 

@@ -47,6 +47,7 @@ SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_uint16), _dp,
                                                   ctypes.c_int32]),
     "m3lb_set_fd_corners": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "m3lb_set_cache_policy": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "m3lb_set_force_correction": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "m3lb_set_outflow_order": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "m3lb_set_outflow_density": (ctypes.c_int, [_vp, ctypes.c_double]),
@@ -277,6 +278,10 @@ class FluidSolver:
         _check(self.L.m3lb_set_regularized_walls(
             self.h, nrm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
             tg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), _d(tab), int(tab.shape[0])))
+
+    def set_cache_policy(self, policy):
+        """abi.CACHE_AUTO / CACHE_STREAM / CACHE_PLAIN (results identical)."""
+        _check(self.L.m3lb_set_cache_policy(self.h, int(policy)))
 
     def set_fd_corners(self, on):
         _check(self.L.m3lb_set_fd_corners(self.h, 1 if on else 0))

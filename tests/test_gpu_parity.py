@@ -237,3 +237,19 @@ def test_device_errors_are_reported():
             s.macroscopic_stored()
         with pytest.raises(RuntimeError):
             s.step(-1)
+
+
+@pytest.mark.parametrize("shape", [(40, 24, 16), (130, 20, 12)])
+def test_cache_policy_is_bit_identical(shape):
+    """m3lb_set_cache_policy changes only how the step kernel's local slots
+    are cached: every policy leaves the same bits."""
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F32, shape, omega=1.3,
+                          force=(1e-5, 0.0, 0.0))
+    out = []
+    for pol in (abi.CACHE_AUTO, abi.CACHE_STREAM, abi.CACHE_PLAIN):
+        with FluidSolver(cfg) as s:
+            s.set_cache_policy(pol)
+            s.initialize_taylor_green(0.04)
+            s.step(13)
+            out.append(s.storage())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])

@@ -188,6 +188,87 @@ def precision_study(D, steps, seed=3):
     return {"size": D, "steps": steps, "seed": seed, "rows": out}
 
 
+def configs_leg():
+    """The other synthetic workloads of the benchmark definition, one line
+    each (not part of the headline):
+      C0  D3Q27 BGK FP64 64^3 Taylor-Green (benchmark form) u0 = 0.05 at
+          Re 1600 on L/(2 pi), seeded U(-1e-4, 1e-4) perturbation (seed 0),
+          500 steps, E and Omega every 50;
+      C1  D3Q27 CM FP32 256^3, u0 = 0.04, Re 1600, seed 1, 1000 steps, mass
+          drift and E every 100;
+      C2  D3Q27 CM FP32 512 x 256 x 256 channel: Solid y and z layers, 200
+          spheres r ~ U[6, 14] (seed 2), Guo g_x = 1e-6, tau = 7/8 (the CM
+          magic value), from rest, 5000 steps; mean u_x = sum rho u_x / sum
+          rho and the momentum-exchange drag on the spheres."""
+    import numpy as np
+    from m3lb_amd import abi, cases
+    from m3lb_amd.solver import FluidSolver
+    out = {}
+    # C0
+    L, u0 = 64, 0.05
+    nu = u0 * L / (2.0 * math.pi * 1600.0)
+    cfg = abi.make_config(abi.D3Q27, abi.OP_BGK, abi.SHIFTED, abi.F64, (L, L, L),
+                          omega=1.0 / (3.0 * nu + 0.5))
+    with FluidSolver(cfg) as s:
+        s.initialize_field(*cases.tgv3d_benchmark(cfg, u0, perturbation=1e-4, seed=0))
+        e0, w0, _ = s.energy_enstrophy()
+        rows, ms = [], 0.0
+        diverged = None
+        for k in range(10):
+            ms += s.time_steps(50)
+            e, w, fin = s.energy_enstrophy()
+            rows.append([50 * (k + 1), e / e0, w / w0])
+            if not fin:
+                diverged = 50 * (k + 1)
+                break
+        out["C0"] = {"steps": rows[-1][0], "mlups": round(L ** 3 * rows[-1][0] / ms / 1e3, 1),
+                     "E_over_E0_and_Omega_over_Omega0": rows, "diverged_at": diverged,
+                     "tau": 3.0 * nu + 0.5}
+    log(f"[bench] C0: {out['C0']['mlups']} MLUPS, final E/E0 {rows[-1][1]:.6f}, "
+        f"diverged {diverged}")
+    # C1
+    L, u0 = 256, 0.04
+    nu = u0 * L / (2.0 * math.pi * 1600.0)
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F32, (L, L, L),
+                          omega=1.0 / (3.0 * nu + 0.5))
+    with FluidSolver(cfg) as s:
+        s.initialize_field(*cases.tgv3d_benchmark(cfg, u0, perturbation=1e-4, seed=1))
+        m0 = s.total_mass()
+        e0, _, _ = s.energy_enstrophy()
+        rows, ms = [], 0.0
+        for k in range(10):
+            ms += s.time_steps(100)
+            e, _, fin = s.energy_enstrophy()
+            rows.append([100 * (k + 1), (s.total_mass() - m0) / m0, e / e0])
+        out["C1"] = {"steps": 1000, "mlups": round(L ** 3 * 1000 / ms / 1e3, 1),
+                     "mass_drift_and_E_over_E0": rows, "finite": bool(fin),
+                     "tau": 3.0 * nu + 0.5}
+    log(f"[bench] C1: {out['C1']['mlups']} MLUPS, mass drift {rows[-1][1]:.2e}")
+    # C2
+    shape = (512, 256, 256)
+    t0 = time.time()
+    flags, body, spheres = cases.sphere_channel(shape, n_spheres=200, rmin=6.0, rmax=14.0, seed=2)
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F32, shape,
+                          periodic=(True, False, False), omega=1.0 / 0.875,
+                          force=(1e-6, 0.0, 0.0))
+    with FluidSolver(cfg) as s:
+        s.set_geometry(flags)
+        s.initialize(1.0)
+        log(f"[bench] C2 geometry: {len(spheres)} spheres, solid fraction "
+            f"{float(np.mean(flags == abi.SOLID)):.4f}, set up in {time.time() - t0:.1f}s")
+        ms = s.time_steps(5000)
+        mass = s.total_mass()
+        mom = s.total_momentum(0)
+        drag = s.momentum_exchange(body)
+        fluid = int(np.sum(flags == abi.FLUID))
+        out["C2"] = {"steps": 5000, "mlups": round(np.prod(shape) * 5000 / ms / 1e3, 1),
+                     "spheres": len(spheres), "solid_fraction": float(np.mean(flags == abi.SOLID)),
+                     "mean_ux": mom / mass, "drag_on_spheres": list(drag),
+                     "body_force_on_fluid": 1e-6 * fluid, "tau": 0.875}
+    log(f"[bench] C2: {out['C2']['mlups']} MLUPS, mean u_x {mom / mass:.4e}, drag x {drag[0]:.4e}")
+    return out
+
+
 def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0):
     from m3lb_amd import cases
     work = cases.serving_workload(n, seed=seed, rate=rate)
@@ -232,6 +313,9 @@ def main():
                          "2.4 M D3Q27 FP32 nodes keep the resident populations in the "
                          "256 MB last-level cache")
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--configs", action="store_true",
+                    help="only run the C0/C1/C2 workloads (Taylor-Green FP64 BGK 64^3, CM "
+                         "FP32 256^3, sphere channel 512x256x256) and print their JSON")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="only run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
     ap.add_argument("--stream-probe-only", type=int, default=0, metavar="BYTES",
@@ -255,6 +339,9 @@ def main():
     if args.precision_study:
         print(json.dumps({"precision": precision_study(args.size, args.precision_study)}),
               flush=True)
+        return
+    if args.configs:
+        print(json.dumps({"configs": configs_leg()}), flush=True)
         return
     if args.serving_only:
         sv = serving_leg(args.serving_requests, args.serving_rate, 4, args.serving_budget_gb,

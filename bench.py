@@ -93,8 +93,12 @@ def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk, max_residen
         log(f"[bench] serving autotune: {tune}")
         t0 = srv.clock() + 0.01
         nxt, ttc, last, peak_res = 0, [], t0, 0
+        next_log = t0 + 20.0
         while len(ttc) < n:
             now = srv.clock()
+            if now > next_log:
+                log(f"[bench] serving batched: {len(ttc)}/{n} complete at {now - t0:.0f} s")
+                next_log = now + 20.0
             while nxt < n and t0 + work[nxt][0] <= now:
                 req = work[nxt][1]
                 req.arrival = t0 + work[nxt][0]
@@ -129,7 +133,11 @@ def serving_per_case(work):
     scfg = abi.make_server_config()
     t0 = time.perf_counter() + 0.01
     free, ttc, step_s = t0, [], 0.0
+    next_log = t0 + 20.0
     for off, r in work:
+        if time.perf_counter() > next_log:
+            log(f"[bench] serving per case: {len(ttc)}/{len(work)} complete")
+            next_log = time.perf_counter() + 20.0
         arr = t0 + off
         now = time.perf_counter()
         if now < arr:

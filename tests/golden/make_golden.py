@@ -1,8 +1,8 @@
 """Extracts the reference's own committed results into tests/golden/reference_data.json.
 
-Run in the build container only (it reads the reference checkout at
-/root/reference as DATA: .dat result files and the measured tables of its
-README).  The GPU box never runs this; tests read the JSON it writes.
+Run where the reference checkout exists (it reads /root/reference as DATA:
+.dat result files and the measured tables of its README).  Tests never run
+it; they read the JSON it writes.
 
 Sources (all in the reference checkout):
   doc/fig/conv_taylor_green_{BGK,TRT,MRT,CM}.dat   grid_convergence.cpp, D2Q9, tau=0.8

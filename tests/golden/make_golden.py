@@ -17,6 +17,7 @@ Sources (all in the reference checkout):
   results/C_cavity/cavity_re*_{d2q9,d3q27}_cm*.dat validation/cavity_cm.cpp, N=129 u_lid=0.02,
                                                    17 Ghia stations and full centrelines
   results/A_poiseuille_inlet/poiseuille_*_cm.dat   validation/poiseuille_inlet.cpp, Ly ladder
+  results/D_dipole/dipole_re*_{d2q9,d3q27}_cm.dat  validation/dipole.cpp, D_lb = 512
 """
 import json
 import os
@@ -98,6 +99,13 @@ def main():
         p = f"results/A_poiseuille_inlet/poiseuille_{lat}_cm.dat"
         pin[lat] = {"file": p, "header": header(p), "rows": rows(p)}
     data["poiseuille_inlet"] = pin
+
+    dip = {}
+    for re_ in (625, 2500):
+        for lat in ("d2q9", "d3q27"):
+            p = f"results/D_dipole/dipole_re{re_}_{lat}_cm.dat"
+            dip[f"re{re_}_{lat}"] = {"file": p, "header": header(p), "rows": rows(p)}
+    data["dipole"] = dip
 
     with open(os.path.join(REF, "README.md")) as f:
         lines = f.read().splitlines()

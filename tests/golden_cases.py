@@ -575,3 +575,50 @@ def poiseuille_inlet_run(solver_cls, lattice, Ly, oo=1, U0=0.005, Re=10.0, Lx=21
     s_ = np.arange(Ly) / (Ly - 1.0)
     ana = 4.0 * U0 * s_ * (1.0 - s_)
     return math.sqrt(float(np.sum((u - ana) ** 2) / np.sum(ana ** 2))), taken, res
+
+
+def dipole_run(solver_cls, lattice, Re, Dlb=512, tmax=0.75, u0=0.02, dtype=abi.F64, **kw):
+    """validation/dipole.cpp:28-140: the dipole-wall collision in the box
+    [-1, 1]^2 of N = 2 Dlb + 1 nodes per side, regularised no-slip walls on
+    the edge nodes (corners marked corner), CM shifted, nu = u0 Dlb / Re,
+    the two Gaussian monopoles (|we| = 299.56, r0 = 0.1, at y = +-0.1) scaled
+    by u0, T = tmax Dlb / u0 steps, E and Omega (Campaign diagnostics) every
+    T / 30 steps.  Returns [(t, E/E0, Omega/Omega0)]."""
+    N = 2 * Dlb + 1
+    nu = u0 * Dlb / Re
+    T = int(tmax * Dlb / u0)
+    dx = 1.0 / Dlb
+    cfg = abi.make_config(lattice, abi.OP_CM, abi.SHIFTED, dtype, (N, N, 1),
+                          periodic=(False, False, True), omega=1.0 / (3.0 * nu + 0.5))
+    z, y, x = np.meshgrid(np.arange(1), np.arange(N), np.arange(N), indexing="ij")
+    xl, xr, yb, yt = x == 0, x == N - 1, y == 0, y == N - 1
+    nrm = np.zeros(x.shape, dtype=np.uint8)
+    nrm[yt] = cases.NRM_YP
+    nrm[yb] = cases.NRM_YM
+    nrm[xr] = cases.NRM_XP
+    nrm[xl] = cases.NRM_XM
+    nrm[(xl | xr) & (yb | yt)] = cases.NRM_CORNER
+    s = solver_cls(cfg, **kw)
+    try:
+        s.set_regularized_walls(nrm.reshape(-1), np.zeros(nrm.size, dtype=np.uint16),
+                                np.array([[0.0, 0.0, 0.0, 1.0]]))
+        px, py, _, d = cases.padded_coords(cfg)
+        X = ((px - d[3]) - Dlb) * dx
+        Y = ((py - d[4]) - Dlb) * dx
+        we, r0, y1, y2 = 299.56, 0.1, 0.1, -0.1
+        ea = np.exp(-(X * X + (Y - y1) ** 2) / (r0 * r0))
+        eb = np.exp(-(X * X + (Y - y2) ** 2) / (r0 * r0))
+        ux = (-0.5 * we * (Y - y1) * ea + 0.5 * we * (Y - y2) * eb) * u0
+        uy = (0.5 * we * X * ea - 0.5 * we * X * eb) * u0
+        s.initialize_field(np.ones_like(ux), ux, uy, np.zeros_like(ux))
+        e0, w0, _ = s.energy_enstrophy()
+        probe = T // 30
+        out = [(0.0, 1.0, 1.0)]
+        for k in range(1, 31):
+            s.step(probe)
+            e, w, ok = s.energy_enstrophy()
+            assert ok, f"diverged at probe {k}"
+            out.append((tmax * k * probe / T, e / e0, w / w0))
+    finally:
+        s.close()
+    return out

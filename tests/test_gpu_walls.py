@@ -431,3 +431,23 @@ def test_specular_node_identity_on_device(lat):
         b, rb = specular_node_3d(FluidSolver, True)
         assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
         assert np.abs(ra - rb).max() <= 1e-12
+
+
+from golden_cases import dipole_run  # noqa: E402
+
+
+@pytest.mark.parametrize("lat", [abi.D2Q9, abi.D3Q27])
+@pytest.mark.parametrize("Re", [625, 2500])
+def test_dipole_wall_collision_against_reference_results(lat, Re):
+    """results/D_dipole/dipole_re<Re>_<lat>_cm.dat (validation/dipole.cpp,
+    D_lb = 512, 1025^2 nodes, 19200 steps): E/E0 and Omega/Omega0 at all 31
+    probes, to the printed 9 digits."""
+    name = f"re{Re}_" + ("d2q9" if lat == abi.D2Q9 else "d3q27")
+    want = GOLD["dipole"][name]["rows"]
+    got = dipole_run(FluidSolver, lat, float(Re))
+    worst = 0.0
+    for (t, e, w), row in zip(got, want):
+        assert abs(t - row[0]) < 1e-6
+        worst = max(worst, abs(e - row[1]) / row[1], abs(w - row[2]) / row[2])
+    print(f"dipole {name}: worst relative deviation {worst:.2e}", flush=True)
+    assert worst <= 1e-8, worst

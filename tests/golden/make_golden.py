@@ -8,7 +8,7 @@ Sources (all in the reference checkout):
   doc/fig/conv_taylor_green_{BGK,TRT,MRT,CM}.dat   grid_convergence.cpp, D2Q9, tau=0.8
   doc/fig/conv_abc_{BGK,TRT,MRT,CM}.dat            grid_convergence.cpp, D3Q27, tau=0.8
   results/A_taylor_green/tgv2d_*.dat               validation/tgv2d.cpp, Re=1000 u0=0.02
-  results/E_tgv3d/tgv3d_re1600_d3q27_cm.dat        validation/tgv3d.cpp, D=64
+  results/E_tgv3d/tgv3d_re{1600,30000}_d3q27_cm.dat validation/tgv3d.cpp, D=64
   README.md, Galilean table                        validation/galilean.cpp
   README.md, Poiseuille slip and L2 tables         validation/poiseuille.cpp
   results/H_forcing_convergence/forcing_conv.dat   validation/forcing_transverse.cpp -conv,
@@ -75,6 +75,8 @@ def main():
 
     p = "results/E_tgv3d/tgv3d_re1600_d3q27_cm.dat"
     data["tgv3d_re1600_d3q27_cm"] = {"file": p, "header": header(p), "rows": rows(p)}
+    p = "results/E_tgv3d/tgv3d_re30000_d3q27_cm.dat"
+    data["tgv3d_re30000_d3q27_cm"] = {"file": p, "header": header(p), "rows": rows(p)}
 
     p = "results/H_forcing_convergence/forcing_conv.dat"
     data["forcing_transverse"] = {"file": p, "header": header(p), "rows": rows(p)}

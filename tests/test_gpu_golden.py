@@ -97,7 +97,22 @@ def test_tgv3d_re1600_file():
         assert abs(t - tg) < 1e-9
         worst = max(worst, rel(E, Eg), rel(W, Wg))
     print(f"tgv3d worst relative difference {worst:.3e}")
-    assert worst < 1e-6
+    assert worst < 6e-9  # the printed 9 digits
+
+
+def test_tgv3d_re30000_file():
+    """results/E_tgv3d/tgv3d_re30000_d3q27_cm.dat: all 20 probes (32000
+    steps at tau = 0.500128, through the transition to turbulence), E/E0 and
+    enstrophy ratio to the printed 9 digits."""
+    rows = GOLD["tgv3d_re30000_d3q27_cm"]["rows"]
+    got = tgv3d_run(FluidSolver, len(rows) - 1, Re=30000.0)
+    devs = []
+    for (t, E, W), (tg, Eg, Wg) in zip(got, rows):
+        assert abs(t - tg) < 1e-9
+        devs.append(max(rel(E, Eg), rel(W, Wg)))
+    print("tgv3d Re 30000 relative differences per probe: " +
+          " ".join(f"{d:.1e}" for d in devs), flush=True)
+    assert max(devs) < 6e-9, devs
 
 
 @pytest.mark.parametrize("row", [r for r in GOLD["forcing_transverse"]["rows"] if r[1] <= 32],

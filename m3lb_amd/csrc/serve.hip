@@ -843,10 +843,13 @@ int m3lb_srv_autotune(m3lb_server* s, int32_t steps, m3lb_tune_result* out) {
   if (steps < 1 || steps > 1024) return err(M3LB_EINVAL, "steps must be in 1..1024");
   if (!idle(s)) return err(M3LB_ESTATE, "autotune needs an idle server");
   // The mix: Taylor-Green boxes, edges log-uniform on [32, 192] from a fixed
-  // sequence, up to max_boxes / 256 boxes, half the budget and 8 GiB.
+  // sequence, up to max_boxes / 256 boxes, half the budget and 8 GiB, and
+  // within the node cap (the resident set the server will run), so that the
+  // tuned regime -- cache-resident or not -- is the one served.
   const uint64_t cap = std::min<uint64_t>(s->arena.capacity() / 2, uint64_t(8) << 30);
+  const uint64_t node_cap = s->cfg.max_resident_nodes;
   std::vector<Box> mix;
-  uint64_t lcg = 0x9E3779B97F4A7C15ull, used = 0;
+  uint64_t lcg = 0x9E3779B97F4A7C15ull, used = 0, used_nodes = 0;
   auto uni = [&]() {
     lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
     return double(lcg >> 11) * (1.0 / 9007199254740992.0);
@@ -868,6 +871,9 @@ int m3lb_srv_autotune(m3lb_server* s, int32_t steps, m3lb_tune_result* out) {
     if (request_domain(s->cfg, &r, b.d) != M3LB_OK) break;
     footprint_of(s->cfg, b.d, b.stride, b.fbytes, b.bytes);
     if (used + b.bytes > cap) break;
+    const uint64_t bn = uint64_t(b.d.nx) * uint64_t(b.d.ny) * uint64_t(b.d.nz);
+    if (node_cap && !mix.empty() && used_nodes + bn > node_cap) break;
+    used_nodes += bn;
     b.off = s->arena.alloc(b.bytes);
     if (b.off == m3lb::Arena::kNone) break;
     used += b.bytes;

@@ -327,7 +327,11 @@ int rebuild_plan(m3lb_solver* s) {
   };
   auto id = [&](int x, int y, int z) { return unsigned(((z + d.hz) * d.sy + (y + d.hy)) * d.sx + x + d.hx); };
   const int D = s->Q == 9 ? 2 : 3;
-  std::vector<int> level(np, 0);
+  // level of each wall, indexed like `walls` (ascending node index)
+  std::vector<int> level(walls.size(), 0);
+  auto wall_pos = [&](int64_t m) {
+    return size_t(std::lower_bound(walls.begin(), walls.end(), unsigned(m)) - walls.begin());
+  };
   std::vector<std::array<int64_t, m3lb::kWallReads>> reads(walls.size());
   int maxlev = 0;
   for (size_t w = 0; w < walls.size(); ++w) {
@@ -365,14 +369,14 @@ int rebuild_plan(m3lb_solver* s) {
     }
     for (int k = 0; k < m3lb::kWallReads; ++k)
       if (r[k] >= 0 && unsigned(r[k]) < n && s->flags_h[size_t(r[k])] == M3LB_REGWALL)
-        level[n] = std::max(level[n], level[size_t(r[k])] + 1);
-    maxlev = std::max(maxlev, level[n]);
+        level[w] = std::max(level[w], level[wall_pos(r[k])] + 1);
+    maxlev = std::max(maxlev, level[w]);
   }
   // wall list by (level, node); rd rows in the same order
   std::vector<size_t> order(walls.size());
   for (size_t w = 0; w < walls.size(); ++w) order[w] = w;
   std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t c) { return level[walls[a]] < level[walls[c]]; });
+                   [&](size_t a, size_t c) { return level[a] < level[c]; });
   std::vector<unsigned> wlist(walls.size()), snap_nodes;
   std::vector<int> rd(walls.size() * m3lb::kWallReads, -1);
   b.level_off.assign(size_t(maxlev) + 2, int(walls.size()));
@@ -380,7 +384,7 @@ int rebuild_plan(m3lb_solver* s) {
     const size_t w = order[k];
     const unsigned n = walls[w];
     wlist[k] = n;
-    const int lv = level[n];
+    const int lv = level[w];
     if (b.level_off[size_t(lv)] > int(k)) b.level_off[size_t(lv)] = int(k);
     for (int r = 0; r < m3lb::kWallReads; ++r) {
       const int64_t m = reads[w][size_t(r)];
@@ -406,12 +410,6 @@ int rebuild_plan(m3lb_solver* s) {
   HIP_OK(hipStreamSynchronize(s->stream));
   b.dirty = false;
   return M3LB_OK;
-}
-
-bool any_boundary_flag(const m3lb_solver* s) {
-  for (uint8_t f : s->flags_h)
-    if (f == M3LB_REGWALL || f == M3LB_SPECWALL || f == M3LB_SPECNODE) return true;
-  return false;
 }
 
 void refresh_geometry(m3lb_solver* s) {

@@ -451,3 +451,20 @@ def test_dipole_wall_collision_against_reference_results(lat, Re):
         worst = max(worst, abs(e - row[1]) / row[1], abs(w - row[2]) / row[2])
     print(f"dipole {name}: worst relative deviation {worst:.2e}", flush=True)
     assert worst <= 1e-8, worst
+
+
+def test_outlet_reading_an_earlier_wall_node():
+    """A back-pressure outlet whose upstream node is itself a wall node (an
+    equilibrium outlet column just before it): the later wall reads the
+    earlier one after its update, so the device runs them in two levels."""
+    shape = (14, 9, 1)
+    cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, abi.F64, shape,
+                          periodic=(False, False, True), omega=1.3)
+    nrm, tag, table = _outflow_walls(shape, cases.NRM_OUT_XP)
+    nrm = nrm.reshape(shape[1], shape[0]).copy()
+    tag = tag.reshape(shape[1], shape[0]).copy()
+    nrm[1:-1, shape[0] - 2] = cases.NRM_OUT_EQ
+    tag[1:-1, shape[0] - 2] = 2
+    s, o = _pair(cfg, (nrm.reshape(-1), tag.reshape(-1), table))
+    with s:
+        _compare(s, o, abi.F64)

@@ -447,9 +447,13 @@ bool is16(const m3lb_solver* s) {
 //   I16: scales proportional to the weights (see measure_scales), the
 //     largest |g| mapped into (2^11, 2^12]: 8x to 16x of headroom under the
 //     saturation at 32767, and a uniform error of half a code.
-// A pair whose largest |g| is below 1e-3 w_q (a field at rest) is scaled as
-// if it were 1e-3 w_q.
-constexpr double kScaleFloor = 1e-3;
+// A pair whose largest |g| is below a floor is scaled as if it were at the
+// floor: 1e-3 w_q for the floating formats, whose range is wide, and 0.1 w_q
+// for I16, whose 8-16x headroom would otherwise be set by a field at rest: a
+// cavity started at rest and driven to u ~ 0.05 (|g| ~ 0.15 w) saturated and
+// lost the flow (velocity error 0.7), while 0.1 w_q leaves room for |g| up to
+// about 0.8 w_q at a resolution of 2.4e-5 w_q.
+double scale_floor(int dtype) { return dtype == M3LB_I16 ? 0.1 : 1e-3; }
 
 double scale_target(int dtype) { return dtype == M3LB_I16 ? 4096.0 : 256.0; }
 
@@ -468,7 +472,7 @@ int floor_log2(double x) {
 }
 
 int scale_exponent(int dtype, int Q, int q, double max_abs) {
-  const double m = std::max(max_abs, kScaleFloor * weight_of(Q, q));
+  const double m = std::max(max_abs, scale_floor(dtype) * weight_of(Q, q));
   return std::max(-60, std::min(60, floor_log2(scale_target(dtype) / m)));
 }
 

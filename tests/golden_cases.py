@@ -442,7 +442,7 @@ def plane_fit(u, H):
 
 
 def specular_node_2d(solver_cls, lattice, mode, M=16, ny=20, steps=300, tau=0.6, U=0.04,
-                     a=0.02, **kw):
+                     a=0.02, dtype=abi.F64, **kw):
     """validation/specular_node.cpp:38-96: a solenoidal mode odd/even about
     x = 0 and x = M, either on the periodic box of width 2M (mode 0) or on
     [0, M] with on-node specular nodes (mode 1, SpecXm / SpecXp) or specular
@@ -450,7 +450,7 @@ def specular_node_2d(solver_cls, lattice, mode, M=16, ny=20, steps=300, tau=0.6,
     (u_x, u_y interleaved over x = 0..M, y = 0..ny-1; rho)."""
     half = mode != 0
     nx = M + 1 if half else 2 * M
-    cfg = abi.make_config(lattice, abi.OP_BGK, abi.SHIFTED, abi.F64, (nx, ny, 1),
+    cfg = abi.make_config(lattice, abi.OP_BGK, abi.SHIFTED, dtype, (nx, ny, 1),
                           periodic=(not half, True, True), omega=1.0 / tau)
     s = solver_cls(cfg, **kw)
     try:

@@ -468,3 +468,46 @@ def test_outlet_reading_an_earlier_wall_node():
     s, o = _pair(cfg, (nrm.reshape(-1), tag.reshape(-1), table))
     with s:
         _compare(s, o, abi.F64)
+
+
+@pytest.mark.parametrize("dtype,tol", [(abi.F16, 2e-3), (abi.BF16, 2e-2), (abi.I16, 2e-3)])
+def test_walls_and_mirrors_in_16_bit_storage(dtype, tol):
+    """Walls, corners and an outlet with 16-bit storage (decoded and
+    re-encoded at their per-direction scales) stay within the storage's
+    rounding of the FP32 run."""
+    N = 24
+    shape = (N, N, 1)
+
+    def run(dt):
+        cfg = abi.make_config(abi.D2Q9, abi.OP_CM, abi.SHIFTED, dt, shape,
+                              periodic=(False, False, True), omega=1.4)
+        nrm, tag, table = cases.cavity_walls(N, 0.05)
+        nrm = nrm.reshape(N, N).copy()
+        nrm[1:-1, N - 1] = cases.NRM_OUT_XP
+        tag = tag.reshape(N, N).copy()
+        tag[1:-1, N - 1] = 2
+        table = np.vstack([table, [[0.0, 0.0, 0.0, 1.0]]])
+        table[:, 3] = 1.0
+        with FluidSolver(cfg) as s:
+            s.set_regularized_walls(nrm.reshape(-1), tag.reshape(-1), table)
+            s.initialize(1.0)
+            s.step(300)
+            _, ux, uy, _ = s.compute_macroscopic()
+        return ux, uy
+
+    ux32, uy32 = run(abi.F32)
+    ux16, uy16 = run(dtype)
+    l2 = np.sqrt(np.sum((ux16 - ux32) ** 2 + (uy16 - uy32) ** 2) / np.sum(ux32 ** 2 + uy32 ** 2))
+    print(f"16-bit walls {dtype}: velocity rel-L2 vs FP32 {l2:.2e}", flush=True)
+    assert l2 < tol, l2
+
+
+@pytest.mark.parametrize("dtype,tol", [(abi.F16, 2e-3), (abi.BF16, 2e-2)])
+def test_specular_nodes_in_16_bit_storage(dtype, tol):
+    """The on-node mirror identity of validation/specular_node.cpp holds in
+    16-bit storage to the storage's rounding."""
+    a, _ = specular_node_2d(FluidSolver, abi.D2Q9, 0)
+    b, _ = specular_node_2d(FluidSolver, abi.D2Q9, 1, dtype=dtype)
+    l2 = np.sqrt(np.sum((a - b) ** 2) / np.sum(a ** 2))
+    print(f"16-bit specular {dtype}: rel-L2 vs FP64 full box {l2:.2e}", flush=True)
+    assert l2 < tol, l2

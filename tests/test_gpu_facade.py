@@ -85,3 +85,23 @@ def test_facade_regularized_walls():
         s.step(steps)
         ux = interior2d(cfg, s.compute_macroscopic()[1])
     np.testing.assert_array_equal(got, ux[:, N // 2] / U)
+
+
+def test_facade_outlets_against_reference_results():
+    """tests/cpp/inlet_facade.cpp: validation/poiseuille_inlet.cpp written
+    against the facade (WallSpec with the outlet's density, the outflow
+    order) reproduces results/A_poiseuille_inlet/poiseuille_d2q9_cm.dat at
+    Ly = 41 and the order-2 figure."""
+    from golden_cases import GOLD
+    exe = os.path.join(os.path.dirname(EXE), "inlet_facade")
+    print("\n[gpu-case] facade inlet Poiseuille", flush=True)
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    row = GOLD["poiseuille_inlet"]["d2q9"]["rows"][0]
+    r = subprocess.run([exe, "41"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    ly, steps, err = r.stdout.split()
+    assert int(ly) == 41 and int(steps) == int(row[2])
+    assert abs(float(err) - row[5]) <= 6e-9 * row[5]
+    r = subprocess.run([exe, "41", "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert abs(float(r.stdout.split()[2]) - 1.388e-3) <= 5e-7

@@ -199,7 +199,8 @@ def precision_study(D, steps, seed=3):
 def configs_leg():
     """The other synthetic workloads of the benchmark definition, one line
     each (not part of the headline):
-      C0  D3Q27 BGK FP64 64^3 Taylor-Green (benchmark form) u0 = 0.05 at
+      C0  D3Q27 BGK (second-order equilibrium) FP64 64^3 Taylor-Green
+          (benchmark form) u0 = 0.05 at
           Re 1600 on L/(2 pi), seeded U(-1e-4, 1e-4) perturbation (seed 0),
           500 steps, E and Omega every 50;
       C1  D3Q27 CM FP32 256^3, u0 = 0.04, Re 1600, seed 1, 1000 steps, mass

@@ -623,3 +623,45 @@ def dipole_run(solver_cls, lattice, Re, Dlb=512, tmax=0.75, u0=0.02, dtype=abi.F
     finally:
         s.close()
     return out
+
+
+def specular_node_octant(solver_cls, octant, M=12, N=10, P=8, steps=300, tau=0.6, U=0.03,
+                         a=0.02, dtype=abi.F64, **kw):
+    """The three-axis counterpart of validation/specular_node.cpp's quarter
+    box: a solenoidal mode with u_x odd in x, u_y odd in y, u_z odd in z (each
+    even in the other two) and rho even in all, on the periodic box 2M x 2N
+    x 2P or on its octant [0, M] x [0, N] x [0, P] with on-node specular
+    faces on all six sides (edges two faces, vertices three).  D3Q27 BGK
+    shifted.  Returns (u, v, w over the octant; rho)."""
+    nx, ny, nz = (M + 1, N + 1, P + 1) if octant else (2 * M, 2 * N, 2 * P)
+    per = not octant
+    cfg = abi.make_config(abi.D3Q27, abi.OP_BGK, abi.SHIFTED, dtype, (nx, ny, nz),
+                          periodic=(per, per, per), omega=1.0 / tau)
+    s = solver_cls(cfg, **kw)
+    try:
+        if octant:
+            z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+            f = np.zeros(x.shape, dtype=np.uint8)
+            for cond, bit in ((x == 0, cases.SPEC_XM), (x == nx - 1, cases.SPEC_XP),
+                              (y == 0, cases.SPEC_YM), (y == ny - 1, cases.SPEC_YP),
+                              (z == 0, cases.SPEC_ZM), (z == nz - 1, cases.SPEC_ZP)):
+                f |= np.where(cond, bit, 0).astype(np.uint8)
+            s.set_specular_nodes(f)
+        px, py, pz, d = cases.padded_coords(cfg)
+        X = math.pi * (px - d[3]) / M
+        Y = math.pi * (py - d[4]) / N
+        Z = math.pi * (pz - d[5]) / P
+        A, B = U * M, U * N
+        C = -(A / M + B / N) * P  # divergence-free
+        s.initialize_field(1.0 + a * np.cos(X) * np.cos(Y) * np.cos(Z),
+                           A * np.sin(X) * np.cos(Y) * np.cos(Z) / max(M, N, P),
+                           B * np.cos(X) * np.sin(Y) * np.cos(Z) / max(M, N, P),
+                           C * np.cos(X) * np.cos(Y) * np.sin(Z) / max(M, N, P))
+        s.step(steps)
+        mac = s.compute_macroscopic()
+        nxp, nyp, nzp, hx, hy, hz, sxp, syp, szp, _ = abi.domain_of(cfg)
+        cut = [np.asarray(q).reshape(szp, syp, sxp)[hz:hz + P + 1, hy:hy + N + 1, hx:hx + M + 1]
+               for q in mac]
+    finally:
+        s.close()
+    return np.stack(cut[1:], axis=-1).reshape(-1), cut[0].reshape(-1)

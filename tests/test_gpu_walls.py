@@ -206,7 +206,7 @@ def test_cavity_against_reference_results(name, lat, Re, restol):
 # Outflow codes, specular cells, wall-state controls: device against oracle
 # ---------------------------------------------------------------------------
 from golden_cases import (plane_fit, poiseuille_inlet_run, specular_channel,  # noqa: E402
-                          specular_node_2d, specular_node_3d)
+                          specular_node_2d, specular_node_3d, specular_node_octant)
 
 
 def _outflow_walls(shape, code, faces=("x",)):
@@ -427,10 +427,11 @@ def test_specular_node_identity_on_device(lat):
     assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
     assert np.abs(ra - rb).max() <= 1e-12
     if lat == abi.D3Q27:
-        a, ra = specular_node_3d(FluidSolver, False)
-        b, rb = specular_node_3d(FluidSolver, True)
-        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
-        assert np.abs(ra - rb).max() <= 1e-12
+        for run in (specular_node_3d, specular_node_octant):
+            a, ra = run(FluidSolver, False)
+            b, rb = run(FluidSolver, True)
+            assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+            assert np.abs(ra - rb).max() <= 1e-12
 
 
 from golden_cases import dipole_run  # noqa: E402

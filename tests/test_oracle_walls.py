@@ -134,7 +134,7 @@ def test_regularized_walls_argument_checks():
 # validation/specular_node.cpp): identities the mirror must satisfy exactly.
 # ---------------------------------------------------------------------------
 from golden_cases import (GOLD, plane_fit, poiseuille_inlet_run, specular_channel,  # noqa: E402
-                          specular_node_2d, specular_node_3d)
+                          specular_node_2d, specular_node_3d, specular_node_octant)
 
 
 @pytest.mark.parametrize("lat,tau", [(abi.D2Q9, 0.8), (abi.D2Q9, 1.2), (abi.D3Q27, 0.8)])
@@ -169,6 +169,16 @@ def test_specular_nodes_compose_at_edges_3d():
     axes (edges carry two faces) equals the full periodic box."""
     a, ra = specular_node_3d(OracleSolver, False)
     b, rb = specular_node_3d(OracleSolver, True)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    assert np.abs(ra - rb).max() <= 1e-12
+
+
+def test_specular_nodes_compose_at_vertices_3d():
+    """Mirrors on all three axes: the octant with specular faces on all six
+    sides (vertices carry three faces) equals the periodic box of twice its
+    extent in every direction."""
+    a, ra = specular_node_octant(OracleSolver, False)
+    b, rb = specular_node_octant(OracleSolver, True)
     assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
     assert np.abs(ra - rb).max() <= 1e-12
 

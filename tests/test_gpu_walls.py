@@ -512,3 +512,48 @@ def test_specular_nodes_in_16_bit_storage(dtype, tol):
     l2 = np.sqrt(np.sum((a - b) ** 2) / np.sum(a ** 2))
     print(f"16-bit specular {dtype}: rel-L2 vs FP64 full box {l2:.2e}", flush=True)
     assert l2 < tol, l2
+
+
+@pytest.mark.parametrize("dtype", [abi.F64, abi.F32])
+def test_every_boundary_kind_together(dtype):
+    """A 3-D duct with all boundary kinds at once: a parabolic velocity
+    inlet, a back-pressure outlet, a regularised floor with corners, an
+    on-node free-slip (specular) ceiling, a Solid block, periodic z, and a
+    per-node Guo force."""
+    nx, ny, nz = 18, 12, 6
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dtype, (nx, ny, nz),
+                          periodic=(False, False, True), omega=1.35, force="field")
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    fl = np.zeros(x.shape, dtype=np.uint8)
+    fl[:, 4:7, 7:10] = abi.SOLID
+    nrm = np.zeros(x.shape, dtype=np.uint8)
+    tag = np.zeros(x.shape, dtype=np.uint16)
+    nrm[y == 0] = cases.NRM_YM
+    inlet = (x == 0) & (y > 0) & (y < ny - 1)
+    nrm[inlet] = cases.NRM_XM
+    tag[inlet] = 1 + (y[inlet] - 1)          # one state per row
+    nrm[(x == nx - 1) & (y > 0) & (y < ny - 1)] = cases.NRM_OUT_XP
+    tag[(x == nx - 1) & (y > 0) & (y < ny - 1)] = ny
+    nrm[((x == 0) | (x == nx - 1)) & ((y == 0) | (y == ny - 1))] = cases.NRM_CORNER
+    table = np.zeros((ny + 1, 4))
+    table[:, 3] = 1.0
+    for r in range(1, ny - 1):
+        s_ = r / (ny - 1.0)
+        table[r, 0] = 0.12 * s_ * (1.0 - s_)
+    table[ny, 3] = 0.998
+    faces = np.zeros(x.shape, dtype=np.uint8)
+    faces[(y == ny - 1) & (x > 0) & (x < nx - 1)] = cases.SPEC_YP
+    px, py, pz, _ = cases.padded_coords(cfg)
+    ff = (1e-6 * np.sin(0.3 * py), 2e-7 * np.cos(0.2 * px), 1e-7 * np.sin(0.5 * pz))
+    o = OracleSolver(cfg)
+    s = FluidSolver(cfg)
+    for q in (s, o):
+        q.set_geometry(fl)
+        q.set_specular_nodes(faces)
+        q.set_regularized_walls(nrm.reshape(-1), tag.reshape(-1), table)
+        q.set_force_field(*ff)
+    field = _field(cfg, o.np)
+    o.initialize_field(*field)
+    s.initialize_field(*field)
+    with s:
+        _compare(s, o, dtype)

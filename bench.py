@@ -79,14 +79,15 @@ def _pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if v else None
 
 
-def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk, max_resident_nodes=0):
+def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk, max_resident_nodes=0,
+                    use_graph=True):
     """The requests through one batched server: arrivals replayed on the
     server clock, continuous admission, completion times from the server."""
     from m3lb_amd import abi
     from m3lb_amd.serve import Server
     scfg = abi.make_server_config(budget_bytes=budget_bytes, max_boxes=4096,
                                   steps_per_chunk=steps_per_chunk,
-                                  max_resident_nodes=max_resident_nodes)
+                                  max_resident_nodes=max_resident_nodes, use_graph=use_graph)
     n = len(work)
     with Server(scfg) as srv:
         tune = srv.autotune(steps=tune_steps) if tune_steps > 0 else None
@@ -278,10 +279,11 @@ def configs_leg():
     return out
 
 
-def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0):
+def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0, use_graph=True):
     from m3lb_amd import cases
     work = cases.serving_workload(n, seed=seed, rate=rate)
-    b = serving_batched(work, int(budget_gb * 1e9), tune_steps, steps_per_chunk, cap)
+    b = serving_batched(work, int(budget_gb * 1e9), tune_steps, steps_per_chunk, cap,
+                        use_graph)
     log(f"[bench] serving batched: {b['mlups']:.0f} MLUPS, p50 {b['p50_s']:.2f} s, "
         f"p99 {b['p99_s']:.2f} s, makespan {b['makespan_s']:.1f} s, peak resident "
         f"{b['peak_resident']}")
@@ -322,6 +324,9 @@ def main():
                          "2.4 M D3Q27 FP32 nodes keep the resident populations in the "
                          "256 MB last-level cache")
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--serving-no-graph", action="store_true",
+                    help="launch the server's chunks directly instead of replaying a "
+                         "captured hipGraph (for kernel-trace profiles)")
     ap.add_argument("--configs", action="store_true",
                     help="only run the C0/C1/C2 workloads (Taylor-Green FP64 BGK 64^3, CM "
                          "FP32 256^3, sphere channel 512x256x256) and print their JSON")
@@ -354,7 +359,8 @@ def main():
         return
     if args.serving_only:
         sv = serving_leg(args.serving_requests, args.serving_rate, 4, args.serving_budget_gb,
-                         args.serving_tune_steps, args.serving_chunk, args.serving_cap)
+                         args.serving_tune_steps, args.serving_chunk, args.serving_cap,
+                         not args.serving_no_graph)
         print(json.dumps({"serving": sv}), flush=True)
         return
     dt = {"f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "i16": abi.I16,
@@ -409,7 +415,7 @@ def main():
     if args.serving_requests > 0:
         serving = serving_leg(args.serving_requests, args.serving_rate, 4,
                               args.serving_budget_gb, args.serving_tune_steps, args.serving_chunk,
-                              args.serving_cap)
+                              args.serving_cap, not args.serving_no_graph)
 
     out = {
         "metric": "D3Q27 CM MLUPS + %HBM roofline at 512^3; batched-solve p50/p99 latency",

@@ -36,7 +36,8 @@ def taylor_green_config(abi, D, dtype, op, storage=None):
 
 
 def bytes_per_node(abi, dtype, Q=27):
-    return 2 * Q * {abi.F64: 8, abi.F32: 4, abi.F16: 2, abi.BF16: 2, abi.I16: 2}[dtype]
+    return 2 * Q * {abi.F64: 8, abi.F32: 4, abi.F16: 2, abi.BF16: 2, abi.I16: 2,
+                    abi.E2M13: 2}[dtype]
 
 
 def gpu_leg(D, steps, warmup, dtype, op, seed=3):
@@ -161,7 +162,7 @@ def serving_per_case(work):
 
 
 def precision_study(D, steps, seed=3):
-    """The Taylor-Green workload stored in FP64, FP32, FP16 and BF16 from the
+    """The Taylor-Green workload stored in FP64, FP32 and every 16-bit form from the
     same initial field: velocity relative L2 and kinetic-energy difference of
     each against the FP64-storage run after `steps` steps, and MLUPS."""
     import numpy as np
@@ -171,7 +172,7 @@ def precision_study(D, steps, seed=3):
     field = cases.tgv3d_benchmark(cfg0, u0, perturbation=1e-4, seed=seed)
     ref, out = None, {}
     for name, dt in (("f64", abi.F64), ("f32", abi.F32), ("f16", abi.F16), ("bf16", abi.BF16),
-                     ("i16", abi.I16)):
+                     ("i16", abi.I16), ("e2m13", abi.E2M13)):
         cfg, _ = taylor_green_config(abi, D, dt, abi.OP_CM)
         with FluidSolver(cfg) as s:
             s.initialize_field(*field)
@@ -179,7 +180,7 @@ def precision_study(D, steps, seed=3):
             ms = s.time_steps(steps)
             _, ux, uy, uz = s.compute_macroscopic()
             e1, w1, fin = s.energy_enstrophy()
-            scales = s.scales() if dt in (abi.F16, abi.BF16, abi.I16) else None
+            scales = s.scales() if dt not in (abi.F64, abi.F32) else None
         row = {"mlups": round(D ** 3 * steps / (ms / 1e3) / 1e6, 1), "energy": e1,
                "energy_ratio_to_initial": e1 / e0, "enstrophy": w1, "finite": bool(fin)}
         if ref is None:
@@ -306,7 +307,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "i16", "f64"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "i16", "e2m13", "f64"])
     ap.add_argument("--cpu-size", type=int, default=192)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -331,7 +332,7 @@ def main():
                     help="only run the C0/C1/C2 workloads (Taylor-Green FP64 BGK 64^3, CM "
                          "FP32 256^3, sphere channel 512x256x256) and print their JSON")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
-                    help="only run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
+                    help="only run the FP64/FP32/16-bit storage study for STEPS steps")
     ap.add_argument("--stream-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain in-place stream of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -364,7 +365,7 @@ def main():
         print(json.dumps({"serving": sv}), flush=True)
         return
     dt = {"f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "i16": abi.I16,
-          "f64": abi.F64}[args.dtype]
+          "e2m13": abi.E2M13, "f64": abi.F64}[args.dtype]
     r = gpu_leg(args.size, args.steps, args.warmup, dt, abi.OP_CM)
     log(f"[bench] gpu {args.size}^3 CM {args.dtype}: {r['mlups']:.0f} MLUPS, "
         f"{r['ms_per_step']:.3f} ms/step, {r['achieved_gbs']:.0f} GB/s, "
@@ -372,7 +373,7 @@ def main():
 
     if args.extra:
         for name, d2, op in (("f16", abi.F16, abi.OP_CM), ("bf16", abi.BF16, abi.OP_CM),
-                             ("i16", abi.I16, abi.OP_CM),
+                             ("i16", abi.I16, abi.OP_CM), ("e2m13", abi.E2M13, abi.OP_CM),
                              ("f64", abi.F64, abi.OP_CM), ("f32-bgk", abi.F32, abi.OP_BGK)):
             x = gpu_leg(args.size, max(10, args.steps // 2), 5, d2, op)
             log(f"[bench] extra {name}: {x['mlups']:.0f} MLUPS {x['achieved_gbs']:.0f} GB/s "

@@ -62,9 +62,15 @@ enum { M3LB_RAW = 0, M3LB_SHIFTED = 1 };
  * 16-bit forms compute in FP32.  The 16-bit forms hold s_q * g_q with one
  * power-of-two scale per opposite pair and require SHIFTED storage: F16 and
  * BF16 as floating point, I16 as fixed point (round to nearest, saturating at
- * +-32767).  The reference selects FP32/FP64 with a configure-time typedef
+ * +-32767), E2M13 as a narrow 16-bit float: bit 0 the sign, bits 15..14 an
+ * exponent e, bits 13..1 a mantissa f, magnitude 2^(4+e) (1 + f/2^13) for
+ * e >= 1 and f 2^-8 for e = 0 (so [0, 256): 14 significant bits over
+ * [32, 256), a step of 2^-8 below), rounded to nearest even, saturating at
+ * the largest code, no infinities or NaNs.
+ * The
+ * reference selects FP32/FP64 with a configure-time typedef
  * (src/core/Types.hpp:16-20) and has no 16-bit form. */
-enum { M3LB_F64 = 0, M3LB_F32 = 1, M3LB_F16 = 2, M3LB_BF16 = 3, M3LB_I16 = 4 };
+enum { M3LB_F64 = 0, M3LB_F32 = 1, M3LB_F16 = 2, M3LB_BF16 = 3, M3LB_I16 = 4, M3LB_E2M13 = 5 };
 
 /* Forcing policy: NoForcing, uniform Guo (src/forcing/Forcing.hpp:41-82), or
  * Guo with a per-node force F(n) set by m3lb_set_force_field -- the form of
@@ -115,7 +121,7 @@ typedef struct m3lb_config {
   int32_t lattice;       /* M3LB_D2Q9 | M3LB_D3Q27 */
   int32_t op;            /* M3LB_OP_* */
   int32_t storage;       /* M3LB_RAW | M3LB_SHIFTED */
-  int32_t dtype;         /* M3LB_F64 | M3LB_F32 | M3LB_F16 | M3LB_BF16 */
+  int32_t dtype;         /* M3LB_F64 | M3LB_F32 | M3LB_F16 | M3LB_BF16 | M3LB_I16 | M3LB_E2M13 */
   int32_t nx, ny, nz;    /* interior extents; nz = 1 for a 2D run */
   int32_t periodic[3];   /* 1 = periodic along x, y, z */
   double omega;          /* BGK omega, TRT omega_plus, moment shear omega */
@@ -277,8 +283,10 @@ int m3lb_get_scales(m3lb_solver* s, double* scales_q);
 int m3lb_set_scales(m3lb_solver* s, const double* scales_q);
 /* Re-chooses the 16-bit scales from the current field: per opposite pair the
  * power of two that maps max |g_q| into (2^7, 2^8] for F16/BF16 (256x
- * headroom under FP16's largest finite value) and (2^11, 2^12] for I16 (8x
- * headroom under saturation); a pair below 1e-3 w_q counts as 1e-3 w_q.
+ * headroom under FP16's largest finite value; a pair below 1e-3 w_q counts
+ * as 1e-3 w_q), and for I16 and E2M13 the power of two that maps r w_q into
+ * (2^11, 2^12] (I16, 8x headroom under saturation) or (32, 64] (E2M13, 4x),
+ * with r the largest max |g_q| / w_q over all directions, at least 0.1.
  * Runs after every initialize; callers may repeat it on a schedule.  No-op
  * for F32/F64. */
 int m3lb_calibrate_scales(m3lb_solver* s);

@@ -219,7 +219,8 @@ enum class Precision : int {
   FP32 = M3LB_F32,
   FP16 = M3LB_F16,
   BF16 = M3LB_BF16,
-  I16 = M3LB_I16
+  I16 = M3LB_I16,
+  E2M13 = M3LB_E2M13
 };
 
 #if defined(LBM_SINGLE_PRECISION)

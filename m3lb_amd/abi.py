@@ -24,6 +24,7 @@ F32 = 1
 F16 = 2
 BF16 = 3
 I16 = 4
+E2M13 = 5
 
 FORCE_NONE = 0
 FORCE_GUO = 1
@@ -41,7 +42,7 @@ ESTATE = -4
 
 OP_NAMES = {OP_NULL: "null", OP_BGK: "bgk", OP_BGK_PRODUCT: "bgk_product",
             OP_TRT: "trt", OP_MRT: "mrt", OP_CM: "cm"}
-DTYPE_NAMES = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16", I16: "i16"}
+DTYPE_NAMES = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16", I16: "i16", E2M13: "e2m13"}
 
 
 class Config(ctypes.Structure):

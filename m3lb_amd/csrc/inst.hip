@@ -50,6 +50,7 @@ M3LB_EXPAND_LOOKUP(M3LB_Q, M3LB_OP) {
   if (dtype == M3LB_F16) return M3LB_PICK2(M3LB_SHIFTED, M3LB_F16);
   if (dtype == M3LB_BF16) return M3LB_PICK2(M3LB_SHIFTED, M3LB_BF16);
   if (dtype == M3LB_I16) return M3LB_PICK2(M3LB_SHIFTED, M3LB_I16);
+  if (dtype == M3LB_E2M13) return M3LB_PICK2(M3LB_SHIFTED, M3LB_E2M13);
   return nullptr;
 #endif
 }

@@ -11,6 +11,7 @@
 #include <utility>
 
 #include "../../include/m3lb.h"
+#include "narrow_float.hpp"
 
 namespace m3lb {
 
@@ -126,6 +127,20 @@ struct Codec<M3LB_I16> {
     const float r = __builtin_rintf(v * s);
     return T(__builtin_fminf(__builtin_fmaxf(r, -32767.0f), 32767.0f));
   }
+};
+
+// ---------------------------------------------------------------------------
+// E2M13: the narrow 16-bit float of narrow_float.hpp with 2 exponent and 13
+// mantissa bits: magnitudes [0, 256), 14 significant bits over [32, 256), a
+// step of 2^-8 below.
+template <>
+struct Codec<M3LB_E2M13> {
+  using T = uint16_t;
+  using R = float;
+  using F = NarrowFloat16<2>;
+  static constexpr bool scaled = true;
+  __device__ __forceinline__ static R dec(T v, float inv) { return F::dec(v) * inv; }
+  __device__ __forceinline__ static T enc(R v, float s) { return F::enc(v * s); }
 };
 
 // ---------------------------------------------------------------------------

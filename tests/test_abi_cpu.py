@@ -73,6 +73,8 @@ def test_domain_matches_oracle():
     dict(dtype=abi.F16, storage=abi.RAW),
     dict(dtype=abi.BF16, op=abi.OP_NULL),
     dict(dtype=abi.I16, storage=abi.RAW),
+    dict(dtype=abi.E2M13, storage=abi.RAW),
+    dict(dtype=6),
     dict(nx=0),
     dict(lattice=abi.D2Q9, nz=4),
     dict(reserved=1),

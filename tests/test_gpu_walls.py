@@ -471,7 +471,8 @@ def test_outlet_reading_an_earlier_wall_node():
         _compare(s, o, abi.F64)
 
 
-@pytest.mark.parametrize("dtype,tol", [(abi.F16, 2e-3), (abi.BF16, 2e-2), (abi.I16, 2e-3)])
+@pytest.mark.parametrize("dtype,tol", [(abi.F16, 2e-3), (abi.BF16, 2e-2), (abi.I16, 2e-3),
+                                       (abi.E2M13, 2e-3)])
 def test_walls_and_mirrors_in_16_bit_storage(dtype, tol):
     """Walls, corners and an outlet with 16-bit storage (decoded and
     re-encoded at their per-direction scales) stay within the storage's
@@ -503,7 +504,7 @@ def test_walls_and_mirrors_in_16_bit_storage(dtype, tol):
     assert l2 < tol, l2
 
 
-@pytest.mark.parametrize("dtype,tol", [(abi.F16, 2e-3), (abi.BF16, 2e-2)])
+@pytest.mark.parametrize("dtype,tol", [(abi.F16, 2e-3), (abi.BF16, 2e-2), (abi.E2M13, 2e-3)])
 def test_specular_nodes_in_16_bit_storage(dtype, tol):
     """The on-node mirror identity of validation/specular_node.cpp holds in
     16-bit storage to the storage's rounding."""

@@ -139,8 +139,11 @@ struct Codec<M3LB_E2M13> {
   using R = float;
   using F = NarrowFloat16<2>;
   static constexpr bool scaled = true;
+  // decoding multiplies by the scale's inverse: the folded form (dec_scaled)
+  // holds two scalar constants per pair, and with them the step kernel's
+  // scheduler serialised its 27 loads to save vector registers
   __device__ __forceinline__ static R dec(T v, float inv) { return F::dec(v) * inv; }
-  __device__ __forceinline__ static T enc(R v, float s) { return F::enc(v * s); }
+  __device__ __forceinline__ static T enc(R v, float s) { return F::enc_scaled(v, s); }
 };
 
 // ---------------------------------------------------------------------------

@@ -53,8 +53,38 @@ struct NarrowFloat16 {
     return uint16_t((m << 1) | (__builtin_bit_cast(uint32_t, x) >> 31));
   }
   M3LB_HD static float dec(uint16_t c) {
-    const float g = __builtin_bit_cast(float, ((uint32_t(c) & 0xfffeu) << (SHIFT - 1)) + BASE);
+    // the sign bit lands below the mantissa's last kept bit and is cleared
+    // after the add (a shift-add is one instruction, a masked shift is not)
+    const float g = __builtin_bit_cast(float, ((uint32_t(c) << (SHIFT - 1)) + BASE) & ~(1u << (SHIFT - 1)));
     const float mag = __builtin_fminf(__builtin_fmaf(g, 2.0f, -P_MIN), g);
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, mag) | (uint32_t(c) << 31));
+  }
+
+  // enc(x * s) and dec(c) * inv for a power-of-two scale s = 1 / inv, with
+  // the scale folded into the constants: multiplying by 2^k adds k to the
+  // exponent field, so G / s = max(|x|, |x| / 2 + 2^(EMIN-1) / s) carries
+  // the bits of G less k << 23 (exactly, while G / s stays a normal float:
+  // |k| <= 60 here).  The constants depend only on s and are uniform over a
+  // wavefront, so they cost scalar integer work, not vector operations.
+  M3LB_HD static uint16_t enc_scaled(float x, float s) {
+    const uint32_t kb = __builtin_bit_cast(uint32_t, s) - 0x3f800000u;  // k << 23
+    const float h = __builtin_bit_cast(float, BASE - kb);               // 2^(EMIN-1) / s
+    const float a = __builtin_fabsf(x);
+    const uint32_t gb = __builtin_bit_cast(uint32_t, __builtin_fmaxf(a, __builtin_fmaf(a, 0.5f, h)));
+    // gb + kb - BASE is the rebased pattern of G; bit SHIFT of it is bit SHIFT
+    // of gb, since kb and BASE are multiples of 2^23
+    uint32_t m = (gb + (kb - BASE) + ((1u << (SHIFT - 1)) - 1u) + ((gb >> SHIFT) & 1u)) >> SHIFT;
+    m = m < MAXM ? m : MAXM;
+    return uint16_t((m << 1) | (__builtin_bit_cast(uint32_t, x) >> 31));
+  }
+  M3LB_HD static float dec_scaled(uint16_t c, float inv) {
+    const uint32_t kb = __builtin_bit_cast(uint32_t, inv) - 0x3f800000u;  // -k << 23
+    // the sign bit lands below the mantissa's last kept bit and is cleared
+    // after the add (shift-add is one instruction, the masked shift is not)
+    const uint32_t gb = ((uint32_t(c) << (SHIFT - 1)) + (BASE + kb)) & ~(1u << (SHIFT - 1));
+    const float g = __builtin_bit_cast(float, gb);
+    const float p = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, P_MIN) + kb);  // 2^EMIN inv
+    const float mag = __builtin_fminf(__builtin_fmaf(g, 2.0f, -p), g);
     return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, mag) | (uint32_t(c) << 31));
   }
 };

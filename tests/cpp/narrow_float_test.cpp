@@ -2,8 +2,10 @@
 // against its definition: every code decodes to the value its fields define,
 // every code re-encodes to itself, encoding a float gives the nearest code
 // with ties to the even code (below the normal binades, within 2^-10 of a
-// step of a tie, either neighbour), and magnitudes beyond the largest code,
-// infinities and NaNs saturate with their sign.  Prints "ok" when all hold.
+// step of a tie, either neighbour), magnitudes beyond the largest code,
+// infinities and NaNs saturate with their sign, and the forms with a
+// power-of-two scale folded in agree with scaling first.  Prints "ok" when
+// all hold.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -73,6 +75,18 @@ void run() {
     check(F::enc(-float(tie)) == code(even, true), "negative tie", double(F::enc(-float(tie))), even);
     check(F::enc(float(tie - band)) == code(m, false), "below tie", double(F::enc(float(tie - band))), m);
     check(F::enc(float(tie + band)) == code(m + 1, false), "above tie", double(F::enc(float(tie + band))), m + 1);
+  }
+  // the scaled forms equal the plain ones at every power-of-two scale used
+  for (int k = -60; k <= 60; k += 7) {
+    const float sc = std::ldexp(1.0f, k), inv = std::ldexp(1.0f, -k);
+    for (uint32_t c = 0; c < 0x10000; c += 7)
+      check(F::dec_scaled(uint16_t(c), inv) == F::dec(uint16_t(c)) * inv, "scaled decode",
+            F::dec_scaled(uint16_t(c), inv), k);
+    for (int j = 0; j < 200000; ++j) {
+      const float x = float(std::exp2(lg(rng)) * ((j & 1) ? -1 : 1)) * inv;
+      check(F::enc_scaled(x, sc) == F::enc(x * sc), "scaled encode", double(F::enc_scaled(x, sc)), k);
+    }
+    check(F::enc_scaled(std::nanf(""), sc) == code(0x7fff, false), "scaled NaN", k, 0);
   }
   const float inf = std::numeric_limits<float>::infinity();
   check(F::enc(inf) == code(0x7fff, false) && F::enc(-inf) == code(0x7fff, true), "infinity saturates", 0, 0);

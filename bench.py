@@ -280,25 +280,37 @@ def configs_leg():
     return out
 
 
-def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0, use_graph=True):
+def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0, use_graph=True,
+                edges=(32, 192)):
     from m3lb_amd import cases
-    work = cases.serving_workload(n, seed=seed, rate=rate)
+    work = cases.serving_workload(n, seed=seed, rate=rate, edges=edges)
     b = serving_batched(work, int(budget_gb * 1e9), tune_steps, steps_per_chunk, cap,
                         use_graph)
     log(f"[bench] serving batched: {b['mlups']:.0f} MLUPS, p50 {b['p50_s']:.2f} s, "
         f"p99 {b['p99_s']:.2f} s, makespan {b['makespan_s']:.1f} s, peak resident "
         f"{b['peak_resident']}")
-    work = cases.serving_workload(n, seed=seed, rate=rate)
+    work = cases.serving_workload(n, seed=seed, rate=rate, edges=edges)
     c = serving_per_case(work)
     log(f"[bench] serving one launch per case: {c['mlups']:.0f} MLUPS, p50 {c['p50_s']:.2f} s, "
         f"p99 {c['p99_s']:.2f} s, makespan {c['makespan_s']:.1f} s")
     rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
     return {"requests": n, "arrival_rate_per_s": rate, "seed": seed,
             "workload": "D3Q27 CM FP32 shifted; 50% periodic Taylor-Green u0=0.04, 50% "
-                        "bounce-back channels g_x~U[1e-7,1e-5]; edges log-uniform [32,192) "
-                        "per axis; tau~U[0.51,0.8]; steps~U{200..3000}; Poisson arrivals",
+                        "bounce-back channels g_x~U[1e-7,1e-5]; edges log-uniform "
+                        f"[{edges[0]},{edges[1]}) "
+                        f"per axis; tau~U[0.51,0.8]; steps~U{{200..3000}}; Poisson arrivals",
             "budget_bytes": int(budget_gb * 1e9), "batched": rnd(b), "per_case": rnd(c),
             "speedup_mlups": round(b["mlups"] / c["mlups"], 3)}
+
+
+def serving_small(args):
+    """The serving comparison on small boxes (edges [16, 48)), where one launch
+    per step per case is launch-bound."""
+    if args.serving_small_requests <= 0 or args.serving_requests <= 0:
+        return None
+    return serving_leg(args.serving_small_requests, args.serving_small_rate, 5,
+                       args.serving_budget_gb, args.serving_tune_steps, args.serving_chunk,
+                       args.serving_cap, not args.serving_no_graph, edges=(16, 48))
 
 
 def main():
@@ -324,6 +336,9 @@ def main():
                     help="max resident interior nodes of the server (0: memory only); "
                          "2.4 M D3Q27 FP32 nodes keep the resident populations in the "
                          "256 MB last-level cache")
+    ap.add_argument("--serving-small-requests", type=int, default=400,
+                    help="requests of the small-box serving workload (edges [16, 48)); 0 skips it")
+    ap.add_argument("--serving-small-rate", type=float, default=400.0, help="arrivals per second")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--no-e2m13", action="store_true",
                     help="skip the E2M13-storage rate of the same workload")
@@ -364,7 +379,8 @@ def main():
         sv = serving_leg(args.serving_requests, args.serving_rate, 4, args.serving_budget_gb,
                          args.serving_tune_steps, args.serving_chunk, args.serving_cap,
                          not args.serving_no_graph)
-        print(json.dumps({"serving": sv}), flush=True)
+        small = serving_small(args)
+        print(json.dumps({"serving": sv, "serving_small_boxes": small}), flush=True)
         return
     dt = {"f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "i16": abi.I16,
           "e2m13": abi.E2M13, "f64": abi.F64}[args.dtype]
@@ -429,6 +445,7 @@ def main():
         serving = serving_leg(args.serving_requests, args.serving_rate, 4,
                               args.serving_budget_gb, args.serving_tune_steps, args.serving_chunk,
                               args.serving_cap, not args.serving_no_graph)
+    serving_small_boxes = serving_small(args)
 
     out = {
         "metric": "D3Q27 CM MLUPS + %HBM roofline at 512^3; batched-solve p50/p99 latency",
@@ -458,6 +475,7 @@ def main():
         "mass_drift": r["mass_drift"],
         "storage_e2m13": narrow,
         "serving": serving,
+        "serving_small_boxes": serving_small_boxes,
     }
     print(json.dumps(out), flush=True)
 

@@ -258,13 +258,14 @@ def channel_flags(shape):
     return fl.reshape(-1)
 
 
-def serving_workload(n, seed=4, rate=40.0, lattice=abi.D3Q27, u0=0.04):
+def serving_workload(n, seed=4, rate=40.0, lattice=abi.D3Q27, u0=0.04, edges=(32, 192)):
     """The batched-serving workload: n independent requests arriving as a
     Poisson stream of `rate` per second (seeded).  Half are periodic
     Taylor-Green boxes (amplitude u0, m3lb_initialize_taylor_green), half
     bounce-back channels (walls at y = 0 and ny - 1, periodic x and z, Guo
     force g_x ~ U[1e-7, 1e-5], starting at rest).  Each edge is drawn
-    separately, log-uniform on [32, 192); tau ~ U[0.51, 0.8]; steps ~ U{200..3000}.
+    separately, log-uniform on [edges[0], edges[1]) (default [32, 192));
+    tau ~ U[0.51, 0.8]; steps ~ U{200..3000}.
     Returns a list of (arrival offset in seconds, abi.Request)."""
     rng = np.random.Generator(np.random.MT19937(seed))
     t = 0.0
@@ -272,7 +273,8 @@ def serving_workload(n, seed=4, rate=40.0, lattice=abi.D3Q27, u0=0.04):
     for _ in range(n):
         t += rng.exponential(1.0 / rate)
         kind = rng.random() < 0.5
-        e = [int(32.0 * 6.0 ** rng.random()) for _ in range(3)]
+        lo, hi = edges
+        e = [int(lo * (hi / lo) ** rng.random()) for _ in range(3)]
         if lattice == abi.D2Q9:
             e[2] = 1
         tau = rng.uniform(0.51, 0.8)

@@ -161,7 +161,10 @@ def serving_per_case(work):
             "p99_s": _pct(ttc, 99), "step_only_mlups": lu / step_s / 1e6}
 
 
-def precision_study(D, steps, seed=3):
+PRECISION_FORMS = ("f64", "f32", "f16", "bf16", "i16", "e2m13")
+
+
+def precision_study(D, steps, seed=3, forms=PRECISION_FORMS):
     """The Taylor-Green workload stored in FP64, FP32 and every 16-bit form from the
     same initial field: velocity relative L2 and kinetic-energy difference of
     each against the FP64-storage run after `steps` steps, and MLUPS."""
@@ -171,8 +174,10 @@ def precision_study(D, steps, seed=3):
     cfg0, u0 = taylor_green_config(abi, D, abi.F64, abi.OP_CM)
     field = cases.tgv3d_benchmark(cfg0, u0, perturbation=1e-4, seed=seed)
     ref, out = None, {}
-    for name, dt in (("f64", abi.F64), ("f32", abi.F32), ("f16", abi.F16), ("bf16", abi.BF16),
-                     ("i16", abi.I16), ("e2m13", abi.E2M13)):
+    dts = {"f64": abi.F64, "f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "i16": abi.I16,
+           "e2m13": abi.E2M13}
+    for name in forms:
+        dt = dts[name]
         cfg, _ = taylor_green_config(abi, D, dt, abi.OP_CM)
         with FluidSolver(cfg) as s:
             s.initialize_field(*field)
@@ -341,7 +346,9 @@ def main():
     ap.add_argument("--serving-small-rate", type=float, default=400.0, help="arrivals per second")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--no-e2m13", action="store_true",
-                    help="skip the E2M13-storage rate of the same workload")
+                    help="skip the E2M13-storage run of the same workload")
+    ap.add_argument("--e2m13-steps", type=int, default=2000,
+                    help="steps of the E2M13 and FP64-storage runs compared in the bench line")
     ap.add_argument("--serving-no-graph", action="store_true",
                     help="launch the server's chunks directly instead of replaying a "
                          "captured hipGraph (for kernel-trace profiles)")
@@ -430,15 +437,18 @@ def main():
             traffic = json.load(fh)["traffic_bytes"]
         traffic_src = "profiles/k_step_cm_f32_512cubed.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
-    # the same workload with E2M13 storage (108 B per node-step; velocity
-    # within 2.0e-4 of FP64 storage after 2000 steps at 512^3)
+    # the same workload with E2M13 storage (108 B per node-step), run for
+    # 2000 steps beside the same run with FP64 storage: rate and error
     narrow = None
     if args.dtype == "f32" and not args.no_e2m13:
-        x = gpu_leg(args.size, args.steps, args.warmup, abi.E2M13, abi.OP_CM)
-        narrow = {"mlups": round(x["mlups"], 1), "vs_f32": round(x["mlups"] / r["mlups"], 3),
-                  "achieved_gbs": round(x["achieved_gbs"], 1), "bytes_per_node_step": 108,
-                  "finite": bool(x["finite"])}
-        log(f"[bench] e2m13 storage: {x['mlups']:.0f} MLUPS ({narrow['vs_f32']}x FP32)")
+        st = precision_study(args.size, args.e2m13_steps, forms=("f64", "e2m13"))
+        x = st["rows"]["e2m13"]
+        narrow = {"mlups": x["mlups"], "vs_f32": round(x["mlups"] / r["mlups"], 3),
+                  "achieved_gbs": round(x["mlups"] * 108 / 1e3, 1), "bytes_per_node_step": 108,
+                  "steps": args.e2m13_steps, "vel_rel_l2_vs_f64": x["vel_rel_l2_vs_f64"],
+                  "energy_rel_diff_vs_f64": x["energy_rel_diff_vs_f64"], "finite": x["finite"]}
+        log(f"[bench] e2m13 storage: {x['mlups']:.0f} MLUPS ({narrow['vs_f32']}x FP32), velocity "
+            f"rel-L2 vs FP64 {x['vel_rel_l2_vs_f64']:.2e} after {args.e2m13_steps} steps")
 
     serving = None
     if args.serving_requests > 0:

@@ -1531,7 +1531,7 @@ __global__ void __launch_bounds__(kBlock) k_rescale(void* f, uint64_t stride, in
     sfor<Q>([&](auto I) {
       constexpr int q = I;
       T& c = A[uint64_t(q) * stride + unsigned(n)];
-      c = C::enc(float(c) * ratio.r[q], 1.0f);
+      c = C::enc(C::dec(c, 1.0f) * ratio.r[q], 1.0f);
     });
   }
 }

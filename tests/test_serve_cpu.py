@@ -108,3 +108,15 @@ def test_null_server_arguments():
     assert L.m3lb_srv_submit(None, None, None, None) == abi.EINVAL
     assert L.m3lb_srv_progress(None, 0, None) == abi.EINVAL
     assert L.m3lb_srv_destroy(None) == abi.OK
+
+
+def test_serving_workload_edges():
+    """The small-box workload draws every edge from its range, and the
+    default range is unchanged (the seeded stream's first request)."""
+    from m3lb_amd import cases
+    small = cases.serving_workload(400, seed=5, rate=400.0, edges=(16, 48))
+    assert all(16 <= e < 48 for _, r in small for e in (r.nx, r.ny, r.nz))
+    assert all(b[0] > a[0] for a, b in zip(small, small[1:]))
+    first = cases.serving_workload(200)[0][1]
+    assert (first.nx, first.ny, first.nz) == (96, 124, 106)
+    assert all(32 <= e < 192 for _, r in cases.serving_workload(200) for e in (r.nx, r.ny, r.nz))

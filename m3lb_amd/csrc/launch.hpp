@@ -39,10 +39,19 @@ inline unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBl
 template <class K>
 struct Launch {
   static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::C::scaled && !K::FIELD;
+  // Occupancy cap of the streaming FP32 step: the kernel uses no LDS, but a
+  // launch that reserves 41000 B of it per block fits at most 3 blocks (12
+  // waves) in a CU's 160 KiB.  D3Q27 CM FP32 at 512^3, interleaved runs:
+  // 25.0 GLUPS at 3 or 4 blocks per CU, 24.8 at 2, 24.4-24.55 uncapped (8
+  // waves per SIMD).  The 16-bit forms lose at low occupancy (FP16 47.5 ->
+  // 41.1, E2M13 44.6 -> 34.0 at 3 blocks), so only 4-byte storage without
+  // geometry on the nontemporal path, the case measured, is capped.
+  static constexpr size_t kStepLdsFp32 = 41000;
   template <int P, bool G>
   static void step_p(const StepArgs& a, unsigned grid, hipStream_t s) {
+    constexpr size_t lds = (sizeof(typename K::T) == 4 && !G) ? kStepLdsFp32 : 0;
     if (a.cached) k_step<K, P, G, false><<<grid, kBlock, 0, s>>>(a);
-    else k_step<K, P, G, true><<<grid, kBlock, 0, s>>>(a);
+    else k_step<K, P, G, true><<<grid, kBlock, lds, s>>>(a);
   }
   static void step(const StepArgs& a, int parity, bool geom, unsigned grid, hipStream_t s) {
     if (parity == 0) {

@@ -40,13 +40,15 @@ def bytes_per_node(abi, dtype, Q=27):
                     abi.E2M13: 2}[dtype]
 
 
-def gpu_leg(D, steps, warmup, dtype, op, seed=3):
+def gpu_leg(D, steps, warmup, dtype, op, seed=3, field=None):
     from m3lb_amd import abi, cases
     from m3lb_amd.solver import FluidSolver
     cfg, u0 = taylor_green_config(abi, D, dtype, op)
     t0 = time.time()
     s = FluidSolver(cfg)
-    s.initialize_field(*cases.tgv3d_benchmark(cfg, u0, perturbation=1e-4, seed=seed))
+    if field is None:
+        field = cases.tgv3d_benchmark(cfg, u0, perturbation=1e-4, seed=seed)
+    s.initialize_field(*field)
     log(f"[bench] init {D}^3 {abi.DTYPE_NAMES[dtype]} op={abi.OP_NAMES[op]} in {time.time() - t0:.1f}s")
     m0 = s.total_mass()
     s.time_steps(warmup)
@@ -63,13 +65,42 @@ def gpu_leg(D, steps, warmup, dtype, op, seed=3):
                 mass_drift=abs(m1 - m0) / m0, finite=finite)
 
 
-def cpu_leg(D, steps, threads):
-    """The oracle (CPU restatement of the reference step) on this host."""
+def host_cpu():
+    """Cores this process may run on, the host's logical CPU count and model."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return usable, os.cpu_count() or 1, model
+
+
+def cpu_threads():
+    """Every core the job is given: the affinity set, bounded by the job's
+    OMP_NUM_THREADS share when the host sets one."""
+    usable, _, _ = host_cpu()
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(usable, int(share))) if share.isdigit() and int(share) > 0 else usable
+
+
+def cpu_leg(D, steps, threads, field=None):
+    """The oracle (CPU restatement of the reference step) on this host, timed
+    as src/app/main.cpp:41-51: one warm-up step, then `steps` steps."""
     from m3lb_amd import abi, cases
     from oracle.binding import OracleSolver
     cfg, u0 = taylor_green_config(abi, D, abi.F32, abi.OP_CM)
     o = OracleSolver(cfg, threads=threads)
-    o.initialize_field(*cases.tgv3d_benchmark(cfg, u0, perturbation=1e-4, seed=3))
+    if field is None:
+        field = cases.tgv3d_benchmark(cfg, u0, perturbation=1e-4, seed=3)
+    o.initialize_field(*field)
     sec = o.time_steps(steps)
     o.close()
     return D ** 3 * steps / sec / 1e6, sec
@@ -325,8 +356,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "i16", "e2m13", "f64"])
-    ap.add_argument("--cpu-size", type=int, default=192)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-size", type=int, default=0,
+                    help="box edge of the CPU baseline (default: the GPU workload's)")
+    ap.add_argument("--cpu-steps", type=int, default=4,
+                    help="timed CPU steps after one warm-up (a bounded sample)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--extra", action="store_true",
@@ -391,7 +424,10 @@ def main():
         return
     dt = {"f32": abi.F32, "f16": abi.F16, "bf16": abi.BF16, "i16": abi.I16,
           "e2m13": abi.E2M13, "f64": abi.F64}[args.dtype]
-    r = gpu_leg(args.size, args.steps, args.warmup, dt, abi.OP_CM)
+    from m3lb_amd import cases
+    cfg0, u00 = taylor_green_config(abi, args.size, abi.F32, abi.OP_CM)
+    field = cases.tgv3d_benchmark(cfg0, u00, perturbation=1e-4, seed=3)
+    r = gpu_leg(args.size, args.steps, args.warmup, dt, abi.OP_CM, field=field)
     log(f"[bench] gpu {args.size}^3 CM {args.dtype}: {r['mlups']:.0f} MLUPS, "
         f"{r['ms_per_step']:.3f} ms/step, {r['achieved_gbs']:.0f} GB/s, "
         f"mass drift {r['mass_drift']:.2e}, finite {r['finite']}")
@@ -406,13 +442,21 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        mlups_cpu, sec = cpu_leg(args.cpu_size, args.cpu_steps, threads)
+        threads = args.cpu_threads or cpu_threads()
+        usable, logical, model = host_cpu()
+        csize = args.cpu_size or args.size
+        mlups_cpu, sec = cpu_leg(csize, args.cpu_steps, threads,
+                                 field=field if csize == args.size else None)
         cpu = {"value": round(mlups_cpu, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-               "sample": f"oracle/ C restatement, D3Q27 CM FP32 shifted, same Taylor-Green "
-                         f"workload at {args.cpu_size}^3, {args.cpu_steps} timed steps after 1 "
-                         f"warm-up ({sec:.1f} s), {threads} threads"}
-        log(f"[bench] cpu {args.cpu_size}^3: {mlups_cpu:.2f} MLUPS on {threads} threads")
+               "host_logical_cpus": logical, "job_cpus": usable, "cpu_model": model,
+               "sample": f"oracle/ C restatement of the reference step (Kokkos Threads "
+                         f"unavailable offline), D3Q27 CM FP32 shifted, the same Taylor-Green "
+                         f"workload at {csize}^3, {args.cpu_steps} timed steps after 1 warm-up "
+                         f"({sec:.1f} s), {threads} threads (every core given to this job; "
+                         f"the host has {logical} logical CPUs, {model})"}
+        log(f"[bench] cpu {csize}^3: {mlups_cpu:.2f} MLUPS on {threads} threads "
+            f"({logical} logical CPUs on the host, {model})")
+    del field
 
     bpn = bytes_per_node(abi, dt)
     # The device's streaming ceiling, measured now on an array the size of the
@@ -429,13 +473,20 @@ def main():
         f"{r['achieved_gbs'] / best_ceiling:.3f} of the better")
 
     # HBM bytes per launch from the committed rocprofv3 --pmc summary of this
-    # exact workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel).
+    # exact workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel),
+    # used only while the loaded library holds the kernels it was measured on.
     traffic, traffic_src = None, None
     tj = os.path.join(ROOT, "profiles", "k_step_cm_f32_512cubed_traffic.json")
+    build = solver.kernel_build_id()
     if args.size == 512 and args.dtype == "f32" and os.path.exists(tj):
         with open(tj) as fh:
-            traffic = json.load(fh)["traffic_bytes"]
-        traffic_src = "profiles/k_step_cm_f32_512cubed.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+            tr = json.load(fh)
+        if tr.get("build_id") == build:
+            traffic = tr["traffic_bytes"]
+            traffic_src = f"{tr['source']} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; build {build})"
+        else:
+            traffic_src = (f"null: the stored count was taken on build {tr.get('build_id')}, "
+                           f"the loaded library is build {build}")
 
     # the same workload with E2M13 storage (108 B per node-step), run for
     # 2000 steps beside the same run with FP64 storage: rate and error
@@ -480,6 +531,7 @@ def main():
                      "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "traffic_source": traffic_src,
+        "kernel_build_id": build,
         "stream_ceiling_gbs": ceil,
         "frac_of_stream_ceiling": round(r["achieved_gbs"] / best_ceiling, 4),
         "mass_drift": r["mass_drift"],

@@ -32,16 +32,18 @@
 extern "C" {
 #endif
 
-/* Velocity sets, esoteric-pull ordering (src/lattice/Lattices.hpp:130-214). */
+/* Velocity sets, esoteric-pull ordering (src/lattice/Lattices.hpp:72-96, 125-159). */
 enum { M3LB_D2Q9 = 9, M3LB_D3Q27 = 27 };
 
 /* Collision operators.
  *   NULL        identity collision, the reference's NullCollision
  *               (tests/test_streaming.cpp:28-38): isolates streaming
- *   BGK         BGK<L, SecondOrderEquilibrium<L>, F, S>   (src/collision/BGK.hpp:169-238)
+ *   BGK         BGK<L, SecondOrderEquilibrium<L>, F, S>   (src/collision/BGK.hpp:31-100)
  *   BGK_PRODUCT BGK<L, HighOrderEquilibrium<L>, F, S>     (Equilibrium.hpp:110-136;
  *               the validation campaign's "bgk", validation/Campaign.hpp:36)
  *   TRT         TRT<L, SecondOrderEquilibrium<L>, F, S>   (src/collision/TRT.hpp:33-134)
+ *   TRT_PRODUCT TRT<L, HighOrderEquilibrium<L>, F, S>     (the same operator with the
+ *               product-form equilibrium; TRT takes any Eq, TRT.hpp:32-33, 86-90)
  *   MRT         MomentCollision<L, F, S, false>           (src/collision/MomentCollision.hpp:380-381)
  *   CM          MomentCollision<L, F, S, true>            (src/collision/MomentCollision.hpp:378-379)
  */
@@ -51,7 +53,8 @@ enum {
   M3LB_OP_BGK_PRODUCT = 2,
   M3LB_OP_TRT = 3,
   M3LB_OP_MRT = 4,
-  M3LB_OP_CM = 5
+  M3LB_OP_CM = 5,
+  M3LB_OP_TRT_PRODUCT = 6
 };
 
 /* Stored variable: f (RawPopulations) or g = f - w (ShiftedPopulations),
@@ -115,7 +118,7 @@ enum {
 };
 
 /* Everything FluidSolver's constructor and the collision policy's members
- * carry (FluidSolver.hpp:42-69; BGK.hpp:181-182; TRT.hpp:46-48;
+ * carry (FluidSolver.hpp:42-69; BGK.hpp:43-44; TRT.hpp:46-48;
  * MomentCollision.hpp:89-91; Domain.hpp:47-56). */
 typedef struct m3lb_config {
   int32_t lattice;       /* M3LB_D2Q9 | M3LB_D3Q27 */
@@ -278,7 +281,9 @@ int m3lb_set_force_field(m3lb_solver* s, const double* fx, const double* fy, con
 int m3lb_set_omega(m3lb_solver* s, double omega, double omega_minus, double omega_bulk);
 
 /* Per-direction scales of 16-bit storage: the stored code is s_q * g_q.
- * Setting them re-encodes the array in place.  Ignored for F32/F64. */
+ * Setting them re-encodes the array in place.  Each scale is a power of two
+ * in 2^-60 .. 2^60, shared by opposite directions (else M3LB_EINVAL).
+ * Ignored for F32/F64. */
 int m3lb_get_scales(m3lb_solver* s, double* scales_q);
 int m3lb_set_scales(m3lb_solver* s, const double* scales_q);
 /* Re-chooses the 16-bit scales from the current field: per opposite pair the
@@ -312,6 +317,36 @@ int m3lb_set_tile_order(m3lb_solver* s, int32_t run);
 enum { M3LB_CACHE_AUTO = 0, M3LB_CACHE_STREAM = 1, M3LB_CACHE_PLAIN = 2 };
 int m3lb_set_cache_policy(m3lb_solver* s, int32_t policy);
 
+/* Two steps in one pass over the population array.  Step t+1 follows step t
+ * a few z-planes behind inside one persistent launch, so that it reads step
+ * t's output from the caches, and a short second launch finishes the rows
+ * whose neighbours another stripe of the box updates: a pair of steps then
+ * moves about the HBM bytes of one.  The result is bit-identical to one launch
+ * per step.  No counterpart in the reference.
+ *   mode  M3LB_FUSION_AUTO (default): pair the steps of FP32 and FP64 boxes
+ *         whose population array exceeds the 256 MB last-level cache (the
+ *         16-bit forms run faster one launch per step);
+ *         M3LB_FUSION_OFF: never; M3LB_FUSION_ON: whenever the box allows it.
+ * A box allows it when it is D3Q27 without a per-node force, has no
+ * regularised walls or specular cells, and nz >= lag + 3.  The tuning values
+ * take 0 for their default: stripes (y-slabs of the box, 1..64, at least 3
+ * rows each; default 8, one per XCD), lag (planes step t+1 trails step t by;
+ * default from the box: 2 at 512^3, 5 at 256^3), blocks_per_cu (1..8;
+ * default 4) and group (256-node tiles per ticket, 1..64; default 1).  A pair
+ * never covers the step that stores the macroscopic fields.  A pass whose
+ * bounded waits give up makes the next synchronising call on the handle fail
+ * with M3LB_EDEVICE. */
+enum { M3LB_FUSION_AUTO = 0, M3LB_FUSION_OFF = 1, M3LB_FUSION_ON = 2 };
+int m3lb_set_step_fusion(m3lb_solver* s, int32_t mode, int32_t stripes, int32_t lag,
+                         int32_t blocks_per_cu, int32_t group);
+/* Pairs of steps the handle has run as one pass since creation. */
+int m3lb_fused_passes(const m3lb_solver* s, uint64_t* passes);
+/* Failure-path test hook: with on != 0 every wait of a two-step pass is made
+ * unsatisfiable and gives up after a few polls, so the pass drains at once
+ * and the next synchronising call reports M3LB_EDEVICE.  Never set outside
+ * tests of that error path; off by default. */
+int m3lb_test_fail_fused_waits(m3lb_solver* s, int32_t on);
+
 /* The block -> tile map the step kernel uses for `grid` blocks and a tile
  * order `run` (host-side copy of the device function): out[b] = tile of block
  * b.  A permutation of [0, grid) for every value; exposed so that this can be
@@ -325,6 +360,13 @@ int m3lb_tile_order_map(uint32_t grid, int32_t run, uint32_t* out);
  * traffic counters.  Allocates and frees its own buffer.  No counterpart in
  * the reference. */
 int m3lb_stream_probe(uint64_t bytes, int32_t nontemporal, int32_t reps, double* best_ms);
+
+/* Identity of the kernels in this library: a hash of every device source
+ * file, this header and the compile flags, fixed at build time.  A stored
+ * measurement (a traffic count, a profile) records the id it was taken on,
+ * and is only reused while the loaded library reports the same id.  No
+ * counterpart in the reference. */
+const char* m3lb_kernel_build_id(void);
 
 /* Device bytes held by the handle (populations + flags + macroscopic). */
 int m3lb_device_bytes(const m3lb_solver* s, uint64_t* bytes);

@@ -30,7 +30,7 @@ namespace m3lb {
 
 using Index = std::int32_t;  // src/core/Types.hpp:28
 
-// ---- lattices (src/lattice/Lattices.hpp:130-214) ----------------------------
+// ---- lattices (src/lattice/Lattices.hpp:72-96, 125-159) ----------------------------
 struct D2Q9 {
   static constexpr const char* name = "D2Q9";
   static constexpr int D = 2, Q = 9, id = M3LB_D2Q9;
@@ -104,7 +104,7 @@ inline double viscosity_from_omega(double w) { return (1.0 / w - 0.5) / 3.0; }
 // ---- collision policies -----------------------------------------------------
 template <class L, class Eq = SecondOrderEquilibrium<L>, class Forcing = NoForcing,
           class Store = RawPopulations>
-struct BGK {  // src/collision/BGK.hpp:169-238
+struct BGK {  // src/collision/BGK.hpp:31-100
   using Lattice = L;
   using Storage = Store;
   using ForcingPolicy = Forcing;
@@ -120,13 +120,12 @@ struct BGK {  // src/collision/BGK.hpp:169-238
 template <class L, class Eq = SecondOrderEquilibrium<L>, class Forcing = NoForcing,
           class Store = RawPopulations>
 struct TRT {  // src/collision/TRT.hpp:33-134
-  static_assert(!Eq::product, "TRT is implemented with the second-order equilibrium only "
-                              "(as the reference's TRT is used)");
   using Lattice = L;
   using Storage = Store;
   using ForcingPolicy = Forcing;
   static constexpr const char* name = "TRT";
-  static constexpr int op = M3LB_OP_TRT;
+  // TRT<L, Eq, F, S> takes either equilibrium (TRT.hpp:32-33, 86-90)
+  static constexpr int op = Eq::product ? M3LB_OP_TRT_PRODUCT : M3LB_OP_TRT;
   static constexpr double magic_3_16 = 3.0 / 16.0;
   double omega_p = 1.0, omega_m = 1.0;
   Forcing forcing{};

@@ -15,6 +15,7 @@ OP_BGK_PRODUCT = 2
 OP_TRT = 3
 OP_MRT = 4
 OP_CM = 5
+OP_TRT_PRODUCT = 6
 
 RAW = 0
 SHIFTED = 1
@@ -25,6 +26,10 @@ F16 = 2
 BF16 = 3
 I16 = 4
 E2M13 = 5
+
+FUSION_AUTO = 0
+FUSION_OFF = 1
+FUSION_ON = 2
 
 FORCE_NONE = 0
 FORCE_GUO = 1
@@ -41,7 +46,7 @@ EDEVICE = -3
 ESTATE = -4
 
 OP_NAMES = {OP_NULL: "null", OP_BGK: "bgk", OP_BGK_PRODUCT: "bgk_product",
-            OP_TRT: "trt", OP_MRT: "mrt", OP_CM: "cm"}
+            OP_TRT: "trt", OP_MRT: "mrt", OP_CM: "cm", OP_TRT_PRODUCT: "trt_product"}
 DTYPE_NAMES = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16", I16: "i16", E2M13: "e2m13"}
 
 

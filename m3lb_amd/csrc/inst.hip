@@ -1,5 +1,5 @@
 // inst.hip -- instantiates every kernel of one (lattice, operator) pair.
-// Compiled once per pair with -DM3LB_Q=<9|27> -DM3LB_OP=<0..5>, so the
+// Compiled once per pair with -DM3LB_Q=<9|27> -DM3LB_OP=<0..6>, so the
 // configurations build in parallel and each object stays small.
 #include "launch.hpp"
 
@@ -7,7 +7,7 @@
 #error "M3LB_Q must be defined (9 or 27)"
 #endif
 #ifndef M3LB_OP
-#error "M3LB_OP must be defined (0..5)"
+#error "M3LB_OP must be defined (0..6)"
 #endif
 
 namespace {

@@ -76,6 +76,7 @@ struct StepArgs {
   int rows;        // ny * nz
   int remap;       // tile order, see tile_of()
   int cached;      // 1: plain accesses everywhere (a cache-resident working set)
+  unsigned lds_cap;  // bytes of LDS a capped step launch reserves (0: none)
   double omega, omega_m, omega_b;
   double F[3];
   void* macro[4];  // rho, ux, uy, uz (compute type), or null
@@ -191,7 +192,7 @@ constexpr bool kNtRemote = false;
 // accesses (the MI355X last-level cache is 256 MB).
 constexpr uint64_t kCacheResidentBytes = 256ull << 20;
 
-// load_rest / load_pair of EsotericPull::Access<P> (EsotericPull.hpp:213-223)
+// load_rest / load_pair of EsotericPull::Access<P> (EsotericPull.hpp:87-97)
 template <class K, int P, bool NTL = kNtLocal>
 __device__ __forceinline__ void load_node(const typename K::T* __restrict__ A, uint64_t stride,
                                           unsigned n, const Offs& o, const float* inv,
@@ -212,7 +213,7 @@ __device__ __forceinline__ void load_node(const typename K::T* __restrict__ A, u
   });
 }
 
-// store_rest / store_pair (EsotericPull.hpp:218-224): writes crossed into the
+// store_rest / store_pair (EsotericPull.hpp:92-98): writes crossed into the
 // very slots the load read.
 template <class K, int P, bool NTL = kNtLocal>
 __device__ __forceinline__ void store_node(typename K::T* __restrict__ A, uint64_t stride,
@@ -533,7 +534,7 @@ __global__ void __launch_bounds__(kBlock) k_seed(SeedArgs a) {
   sfor<Q>([&](auto Qs) {
     constexpr int q = Qs;
     // direction i written into slot q, and the displacement e with
-    // nbr(writer, e) == m for remote slots (EsotericPull.hpp:230-237)
+    // nbr(writer, e) == m for remote slots (EsotericPull.hpp:104-111)
     int i = q, ex = 0, ey = 0, ez = 0;
     bool remote = false;
     if constexpr (q != 0) {
@@ -578,7 +579,7 @@ __global__ void __launch_bounds__(kBlock) k_seed(SeedArgs a) {
 
 // ---------------------------------------------------------------------------
 // Logical readout: gather<P> (FluidSolver.hpp:1243-1253) through
-// Access<P>::load (EsotericPull.hpp:191-198) with the reference's neighbour
+// Access<P>::load (EsotericPull.hpp:65-72) with the reference's neighbour
 // rule, clamping included, for every padded node.
 // ---------------------------------------------------------------------------
 struct ReadArgs {
@@ -1514,8 +1515,12 @@ __global__ void __launch_bounds__(kBlock) k_absmax(ReadArgs a) {
   });
 }
 
-// Re-encode 16-bit storage from old to new scales (powers of two: exact unless
-// the new code over- or underflows).
+// Re-encode 16-bit storage from old to new scales.  The ratio is a power of
+// two, so a larger scale is exact unless the code saturates.  A smaller scale
+// drops low-order bits wherever the form has a fixed step: every I16 code, and
+// E2M13 codes below 32 (step 2^-8) and FP16 subnormals.  Those values are
+// rounded to the nearest code of the new scale, an error of at most half its
+// step; the floating forms' normal codes stay exact.
 struct Ratio {
   float r[27];
 };

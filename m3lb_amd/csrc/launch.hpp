@@ -7,6 +7,7 @@
 #pragma once
 
 #include "kernels.hpp"
+#include "pair.hpp"
 
 namespace m3lb {
 
@@ -30,6 +31,10 @@ struct KernelSet {
   void (*specular)(const SpecArgs&, int parity, hipStream_t);
   // batched server step; null for the identity op and 16-bit storage
   void (*batched)(const BatchArgs&, unsigned grid, hipStream_t);
+  // two steps in one pass (pair.hpp): launch 1 on a persistent grid, then the
+  // deferred rows; null where not instantiated (D2Q9, per-node force)
+  void (*pair)(const PairArgs&, int parity, bool geom, unsigned grid, unsigned tail_grid,
+               unsigned nclear, unsigned* host_err, hipStream_t);
   int elem_bytes;    // bytes per stored population
   int real_bytes;    // bytes of the compute type
 };
@@ -39,17 +44,20 @@ inline unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBl
 template <class K>
 struct Launch {
   static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::C::scaled && !K::FIELD;
-  // Occupancy cap of the streaming FP32 step: the kernel uses no LDS, but a
-  // launch that reserves 41000 B of it per block fits at most 3 blocks (12
-  // waves) in a CU's 160 KiB.  D3Q27 CM FP32 at 512^3, interleaved runs:
-  // 25.0 GLUPS at 3 or 4 blocks per CU, 24.8 at 2, 24.4-24.55 uncapped (8
-  // waves per SIMD).  The 16-bit forms lose at low occupancy (FP16 47.5 ->
-  // 41.1, E2M13 44.6 -> 34.0 at 3 blocks), so only 4-byte storage without
-  // geometry on the nontemporal path, the case measured, is capped.
-  static constexpr size_t kStepLdsFp32 = 41000;
+  // Occupancy cap of the streaming step: the kernel uses no LDS, but a launch
+  // that reserves a.lds_cap bytes of it per block (a quarter of the CU's LDS
+  // plus one byte, set up by the solver from the device's attributes) fits at
+  // most 3 blocks (12 waves) per CU.  Measured for D3Q27 CM FP32 at 512^3,
+  // interleaved runs: 25.0 GLUPS at 3 or 4 blocks per CU, 24.8 at 2,
+  // 24.4-24.55 uncapped (8 waves per SIMD).  The 16-bit forms lose at low
+  // occupancy (FP16 47.5 -> 41.1, E2M13 44.6 -> 34.0 at 3 blocks), so only
+  // the measured case is capped: D3Q27 CM with 4-byte storage, no geometry
+  // and no per-node force, on the nontemporal path.
+  static constexpr bool kCapped = sizeof(typename K::T) == 4 && K::Q == 27 &&
+                                  K::OP == M3LB_OP_CM && !K::FIELD;
   template <int P, bool G>
   static void step_p(const StepArgs& a, unsigned grid, hipStream_t s) {
-    constexpr size_t lds = (sizeof(typename K::T) == 4 && !G) ? kStepLdsFp32 : 0;
+    const size_t lds = (kCapped && !G) ? a.lds_cap : 0;
     if (a.cached) k_step<K, P, G, false><<<grid, kBlock, 0, s>>>(a);
     else k_step<K, P, G, true><<<grid, kBlock, lds, s>>>(a);
   }
@@ -117,6 +125,25 @@ struct Launch {
   static void exchange(const ReadArgs& a, const uint8_t* body, unsigned grid, hipStream_t s) {
     k_momentum_exchange<K><<<grid, kBlock, 0, s>>>(a, body);
   }
+  static constexpr bool kPair = K::Q == 27 && !K::FIELD;
+  template <int P, bool G>
+  static void pair_p(const PairArgs& a, unsigned grid, unsigned tail_grid, unsigned nclear,
+                     unsigned* host_err, hipStream_t s) {
+    k_pair<K, P, G><<<grid, kBlock, 0, s>>>(a);
+    k_pair_tail<K, P, G><<<tail_grid, kBlock, 0, s>>>(a, nclear, host_err);
+  }
+  static void pair(const PairArgs& a, int parity, bool geom, unsigned grid, unsigned tail_grid,
+                   unsigned nclear, unsigned* host_err, hipStream_t s) {
+    if constexpr (kPair) {
+      if (parity == 0) {
+        if (geom) pair_p<0, true>(a, grid, tail_grid, nclear, host_err, s);
+        else pair_p<0, false>(a, grid, tail_grid, nclear, host_err, s);
+      } else {
+        if (geom) pair_p<1, true>(a, grid, tail_grid, nclear, host_err, s);
+        else pair_p<1, false>(a, grid, tail_grid, nclear, host_err, s);
+      }
+    }
+  }
   static void batched(const BatchArgs& a, unsigned grid, hipStream_t s) {
     if constexpr (kBatched) {
       if (a.run == kRunBlockRanges) {
@@ -134,6 +161,7 @@ struct Launch {
                                 rescale,  exchange, wall_density, walls,
                                 snapshot, specular,
                                 kBatched ? batched : nullptr,
+                                kPair ? pair : nullptr,
                                 int(sizeof(typename K::T)), int(sizeof(typename K::R))};
     return &t;
   }

@@ -29,7 +29,7 @@ __host__ __device__ __forceinline__ void sfor(F&& f) {
 
 // ---------------------------------------------------------------------------
 // Velocity sets in esoteric-pull order: rest at 0, opposite pairs (i, i+1) for
-// odd i (src/lattice/Lattices.hpp:130-214).
+// odd i (src/lattice/Lattices.hpp:72-96, 125-159).
 // ---------------------------------------------------------------------------
 template <int Q>
 struct Lat;
@@ -148,7 +148,7 @@ struct Codec<M3LB_E2M13> {
 
 // ---------------------------------------------------------------------------
 // Kernel configuration: everything the reference makes a template parameter
-// of FluidSolver / the collision policy (FluidSolver.hpp:34-35, BGK.hpp:169-170,
+// of FluidSolver / the collision policy (FluidSolver.hpp:34-35, BGK.hpp:31-32,
 // MomentCollision.hpp:65-66).
 // ---------------------------------------------------------------------------
 template <int Q_, int OP_, int ST_, int DT_, int FORCE_>
@@ -171,7 +171,7 @@ struct Params {
 
 template <class R>
 struct Macro {
-  R dens, ux, uy, uz;  // dens = rho (raw) or rho - 1 (shifted), BGK.hpp:164-167
+  R dens, ux, uy, uz;  // dens = rho (raw) or rho - 1 (shifted), BGK.hpp:26-29
 };
 
 template <class K>
@@ -180,7 +180,7 @@ __device__ __forceinline__ typename K::R weight(int i) {
   return R(Lat<K::Q>::wn[i]) / R(Lat<K::Q>::wden);
 }
 
-// Collision::density, BGK.hpp:191-194
+// Collision::density, BGK.hpp:53-56
 template <class K>
 __device__ __forceinline__ typename K::R density(const Macro<typename K::R>& m) {
   using R = typename K::R;
@@ -191,7 +191,7 @@ __device__ __forceinline__ typename K::R density(const Macro<typename K::R>& m) 
 // Raw monomial moments S_pqr = sum_i c_ix^p c_iy^q c_iz^r f_i of order <= 2,
 // from one factorised pass: sums and differences along z, then y, then x
 // (the first-order part of ProductBasis::to_moments at u_b = 0,
-// ProductBasis.hpp:295-317).  Unused moments are dead code after inlining.
+// ProductBasis.hpp:159-181).  Unused moments are dead code after inlining.
 template <class R>
 struct RawMoments {
   R S0, Sx, Sy, Sz, Sxx, Syy, Szz, Sxy, Sxz, Syz;
@@ -243,14 +243,14 @@ __device__ __forceinline__ RawMoments<typename K::R> raw_moments(const typename 
   return S;
 }
 
-// rho and u from S0 and S_a (BGK.hpp:201-216) with Guo::shift_velocity
+// rho and u from S0 and S_a (BGK.hpp:63-78) with Guo::shift_velocity
 // (Forcing.hpp:65-69).
 template <class K>
 __device__ __forceinline__ Macro<typename K::R> macro_of(const RawMoments<typename K::R>& S,
                                                          const Params<typename K::R>& p);
 
 // macroscopic(): sum f, sum c f, u = m / rho, Guo half shift
-// (BGK.hpp:201-216, Forcing.hpp:65-69); NullCollision gives {1,0,0,0}.
+// (BGK.hpp:63-78, Forcing.hpp:65-69); NullCollision gives {1,0,0,0}.
 template <class K>
 __device__ __forceinline__ Macro<typename K::R> macroscopic(const typename K::R* f,
                                                             const Params<typename K::R>& p) {
@@ -341,7 +341,7 @@ __device__ __forceinline__ typename K::R guo_raw(const Macro<typename K::R>& m,
 }
 
 // ---------------------------------------------------------------------------
-// BGK (BGK.hpp:219-230) and TRT (TRT.hpp:92-127).
+// BGK (BGK.hpp:81-92) and TRT (TRT.hpp:92-127).
 // ---------------------------------------------------------------------------
 template <class K, bool PRODUCT>
 __device__ __forceinline__ void collide_bgk(typename K::R* f, const Macro<typename K::R>& m,
@@ -357,7 +357,9 @@ __device__ __forceinline__ void collide_bgk(typename K::R* f, const Macro<typena
   });
 }
 
-template <class K>
+// TRT<L, Eq, F, S> with either equilibrium (TRT.hpp:32-33, 86-90): PRODUCT
+// selects the product form, as for BGK.
+template <class K, bool PRODUCT>
 __device__ __forceinline__ void collide_trt(typename K::R* f, const Macro<typename K::R>& m,
                                             const Params<typename K::R>& p) {
   using R = typename K::R;
@@ -365,13 +367,13 @@ __device__ __forceinline__ void collide_trt(typename K::R* f, const Macro<typena
   const R cp = R(0.5) * (R(1) - R(0.5) * wp);
   const R cm = R(0.5) * (R(1) - R(0.5) * wm);
   {
-    const R e0 = equilibrium<K, false, 0>(m);
+    const R e0 = equilibrium<K, PRODUCT, 0>(m);
     f[0] += wp * (e0 - f[0]);
     if constexpr (K::FORCE) f[0] += R(2) * cp * guo_raw<K, 0>(m, p);
   }
   sfor<(K::Q - 1) / 2>([&](auto P) {
     constexpr int i = 2 * P + 1, j = i + 1;
-    const R ei = equilibrium<K, false, i>(m), ej = equilibrium<K, false, j>(m);
+    const R ei = equilibrium<K, PRODUCT, i>(m), ej = equilibrium<K, PRODUCT, j>(m);
     R dp = R(0.5) * wp * ((ei + ej) - (f[i] + f[j]));
     R dm = R(0.5) * wm * ((ei - ej) - (f[i] - f[j]));
     if constexpr (K::FORCE) {
@@ -386,7 +388,7 @@ __device__ __forceinline__ void collide_trt(typename K::R* f, const Macro<typena
 
 // ---------------------------------------------------------------------------
 // Factorised product basis {1, C, C^2 - cs2}, C = c - u_b
-// (src/collision/ProductBasis.hpp:273-341).  Moment slot of (p, q, r) is
+// (src/collision/ProductBasis.hpp:137-205).  Moment slot of (p, q, r) is
 // 9p + 3q + r on D3Q27 and 3p + q on D2Q9.
 // ---------------------------------------------------------------------------
 template <int D>
@@ -565,7 +567,7 @@ __device__ __forceinline__ void collide_moment(typename K::R* f, const Macro<typ
 // moments S_pqr = sum c_x^p c_y^q c_z^r f (p+q+r <= 2) come out of one
 // factorised pass (sums and differences along z, then y, then x), give rho and
 // u, and are shifted to the Hermite basis {1, C, C^2 - cs2} at u_b.  Exactly
-// the first ten moments of ProductBasis::to_moments (ProductBasis.hpp:295-317);
+// the first ten moments of ProductBasis::to_moments (ProductBasis.hpp:159-181);
 // the moments of order >= 3 it would also produce are overwritten by their
 // equilibria (MomentCollision.hpp:287-288), so they are never formed.
 template <class K, bool CENTRAL>
@@ -714,12 +716,13 @@ __device__ __forceinline__ void collide(typename K::R* f, const Macro<typename K
                                         const Params<typename K::R>& p) {
   if constexpr (K::OP == M3LB_OP_BGK) collide_bgk<K, false>(f, m, p);
   else if constexpr (K::OP == M3LB_OP_BGK_PRODUCT) collide_bgk<K, true>(f, m, p);
-  else if constexpr (K::OP == M3LB_OP_TRT) collide_trt<K>(f, m, p);
+  else if constexpr (K::OP == M3LB_OP_TRT) collide_trt<K, false>(f, m, p);
+  else if constexpr (K::OP == M3LB_OP_TRT_PRODUCT) collide_trt<K, true>(f, m, p);
   else if constexpr (K::OP == M3LB_OP_MRT) collide_moment<K, false>(f, m, p);
   else if constexpr (K::OP == M3LB_OP_CM) collide_moment<K, true>(f, m, p);
 }
 
-// seed_value for all Q directions: BGK.hpp:233-237, TRT.hpp:129-133,
+// seed_value for all Q directions: BGK.hpp:95-99, TRT.hpp:129-133,
 // MomentCollision.hpp:320-330 (inverse transform of the raw-basis equilibrium
 // moments), NullCollision tests/test_streaming.cpp:37.
 template <class K>
@@ -781,7 +784,7 @@ __device__ __forceinline__ void seed(typename K::R* out, typename K::R rho, type
     Macro<R> m;
     m.dens = K::SHIFTED ? rho - R(1) : rho;
     m.ux = ux; m.uy = uy; m.uz = uz;
-    constexpr bool PRODUCT = K::OP == M3LB_OP_BGK_PRODUCT;
+    constexpr bool PRODUCT = K::OP == M3LB_OP_BGK_PRODUCT || K::OP == M3LB_OP_TRT_PRODUCT;
     sfor<Q>([&](auto I) { out[int(I)] = equilibrium<K, PRODUCT, I>(m); });
   }
 }

@@ -140,7 +140,7 @@ int validate(const m3lb_server_config* c) {
   if (!c) return err(M3LB_EINVAL, "config is null");
   if (c->lattice != M3LB_D2Q9 && c->lattice != M3LB_D3Q27)
     return err(M3LB_EINVAL, "lattice must be M3LB_D2Q9 or M3LB_D3Q27");
-  if (c->op < M3LB_OP_BGK || c->op > M3LB_OP_CM)
+  if (c->op < M3LB_OP_BGK || c->op > M3LB_OP_TRT_PRODUCT)
     return err(M3LB_EINVAL, "the server runs the collision operators BGK .. CM");
   if (c->storage != M3LB_RAW && c->storage != M3LB_SHIFTED)
     return err(M3LB_EINVAL, "storage must be M3LB_RAW or M3LB_SHIFTED");
@@ -298,6 +298,35 @@ int seed_box(m3lb_server* s, Box& b) {
   return launch_ok();
 }
 
+void free_box_memory(m3lb_server* s, Box& b) {
+  if (b.off != m3lb::Arena::kNone) s->arena.release(b.off, b.bytes);
+  b.off = m3lb::Arena::kNone;
+}
+
+void drop_resident(m3lb_server* s, int64_t id) {
+  auto it = std::find(s->resident.begin(), s->resident.end(), id);
+  if (it == s->resident.end()) return;
+  s->resident.erase(it);
+  const m3lb::Dom& d = s->boxes.at(id).d;
+  s->resident_nodes -= uint64_t(d.nx) * uint64_t(d.ny) * uint64_t(d.nz);
+}
+
+// A resident box leaves: its slot and memory are free for the next admission
+// and it is reported cancelled.  Launches already issued may still step it;
+// they are ordered before anything a later owner of its memory or slot issues.
+void cancel_resident(m3lb_server* s, int64_t id) {
+  Box& b = s->boxes.at(id);
+  drop_resident(s, id);
+  s->free_slots.push_back(b.slot);
+  b.slot = -1;
+  free_box_memory(s, b);
+  b.steps_left = 0;
+  b.state = M3LB_REQ_CANCELLED;
+  b.t_done = now_s(s);
+  s->completed.push_back(id);
+  ++s->st.cancelled;
+}
+
 int admit(m3lb_server* s) {
   if (s->adm.size() == 0 || s->free_slots.empty()) return M3LB_OK;
   int rc = M3LB_OK;
@@ -325,8 +354,12 @@ int admit(m3lb_server* s) {
     b.parity = 0;
     describe(s, b);
     s->resident.push_back(id);
-    if (!rc) rc = seed_box(s, b);
   }
+  // A box whose seeding fails, and every box admitted after it, is cancelled:
+  // its slot and memory go back, and nothing steps it.
+  rc = m3lb::seed_admitted(
+      ids, [&](int64_t id) { return seed_box(s, s->boxes.at(id)); },
+      [&](int64_t id) { cancel_resident(s, id); });
   return rc;
 }
 
@@ -438,18 +471,6 @@ int issue(m3lb_server* s, bool* issued) {
   return M3LB_OK;
 }
 
-void free_box_memory(m3lb_server* s, Box& b) {
-  if (b.off != m3lb::Arena::kNone) s->arena.release(b.off, b.bytes);
-  b.off = m3lb::Arena::kNone;
-}
-
-void drop_resident(m3lb_server* s, int64_t id) {
-  auto it = std::find(s->resident.begin(), s->resident.end(), id);
-  if (it == s->resident.end()) return;
-  s->resident.erase(it);
-  const m3lb::Dom& d = s->boxes.at(id).d;
-  s->resident_nodes -= uint64_t(d.nx) * uint64_t(d.ny) * uint64_t(d.nz);
-}
 
 // Retires the oldest chunk if it has finished (or wait: once it finishes).
 int retire_one(m3lb_server* s, bool wait, bool* done) {
@@ -797,15 +818,12 @@ int m3lb_srv_cancel(m3lb_server* s, int64_t id) {
   Box* b;
   int rc = find_box(s, id, &b);
   if (rc) return rc;
+  if (b->state == M3LB_REQ_RESIDENT) {
+    cancel_resident(s, id);
+    return M3LB_OK;
+  }
   if (b->state == M3LB_REQ_QUEUED) {
     s->adm.remove(id);
-  } else if (b->state == M3LB_REQ_RESIDENT) {
-    // Launches already issued may still step the box; they are ordered before
-    // anything a later owner of its memory or slot issues.
-    drop_resident(s, id);
-    s->free_slots.push_back(b->slot);
-    b->slot = -1;
-    free_box_memory(s, *b);
   } else {
     return err(M3LB_ESTATE, "request already completed or cancelled");
   }

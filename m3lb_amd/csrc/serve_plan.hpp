@@ -4,7 +4,9 @@
 //              256-byte granularity, freed ranges coalesce with their neighbours;
 //   Admission  the order in which queued requests are admitted: first come,
 //              first served, with a bounded number of smaller requests allowed
-//              past a queue head that does not fit yet (backfill).
+//              past a queue head that does not fit yet (backfill);
+//   seed_admitted  seeds the boxes of one admission pass and, at the first
+//              failure, hands that box and every later one back unseeded.
 // The reference has no serving layer; nothing here has a counterpart in it.
 #pragma once
 
@@ -137,5 +139,21 @@ class Admission {
   int max_bypass_;
   int bypass_ = 0;  // admissions past the current head since it became head
 };
+
+// Seeds the boxes admitted in one pass, in admission order.  seed(id) returns
+// 0 or an error code.  At the first failure undo(id) is called for that box and
+// then for every later one (none of which has been seeded), so no box is left
+// resident on memory it never initialised, and the error is returned.
+template <class Seed, class Undo>
+int seed_admitted(const std::vector<int64_t>& ids, Seed&& seed, Undo&& undo) {
+  for (size_t k = 0; k < ids.size(); ++k) {
+    const int rc = seed(ids[k]);
+    if (rc != 0) {
+      for (size_t j = k; j < ids.size(); ++j) undo(ids[j]);
+      return rc;
+    }
+  }
+  return 0;
+}
 
 }  // namespace m3lb

@@ -24,13 +24,19 @@ from . import abi
 
 
 def write_dat(path, header, rows, fmt="%.8e"):
-    """header: column names (list) or the full header text without '# '."""
+    """header: column names (list) or the full header text without '# '.
+    fmt: one printf format for every column, or one per column (the
+    campaign's tables print t* as %.6f and the ratios as %.8e,
+    validation/tgv3d.cpp:79-92)."""
     if not isinstance(header, str):
         header = " ".join(header)
     with open(path, "w") as f:
         f.write("# " + header + "\n")
         for r in rows:
-            f.write(" ".join(fmt % float(v) for v in r) + "\n")
+            fm = [fmt] * len(r) if isinstance(fmt, str) else list(fmt)
+            if len(fm) != len(r):
+                raise ValueError(f"write_dat: {len(fm)} formats for a row of {len(r)} values")
+            f.write(" ".join(c % float(v) for c, v in zip(fm, r)) + "\n")
 
 
 def read_dat(path):

@@ -1,7 +1,7 @@
-"""Velocity sets in esoteric-pull order (src/lattice/Lattices.hpp:130-214).
+"""Velocity sets in esoteric-pull order (src/lattice/Lattices.hpp:72-96, 125-159).
 
 Index 0 is the rest velocity; (1,2), (3,4), ... are opposite pairs, so
-opp(i) = i+1 for odd i and i-1 for even i (Lattices.hpp:98-100).  Weights are
+opp(i) = i+1 for odd i and i-1 for even i (Lattices.hpp:40-45).  Weights are
 exact rationals w_num / w_den.
 """
 import numpy as np

@@ -67,6 +67,11 @@ SIGNATURES = {
     "m3lb_tile_order_map": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int32,
                                            ctypes.POINTER(ctypes.c_uint32)]),
     "m3lb_stream_probe": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _dp]),
+    "m3lb_kernel_build_id": (ctypes.c_char_p, []),
+    "m3lb_set_step_fusion": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32, ctypes.c_int32]),
+    "m3lb_fused_passes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "m3lb_test_fail_fused_waits": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "m3lb_initialize_taylor_green": (ctypes.c_int, [_vp, ctypes.c_double]),
     "m3lb_momentum_exchange": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8), _dp]),
     # batched server
@@ -98,6 +103,11 @@ def stream_probe(nbytes, nontemporal=True, reps=5):
     ms = ctypes.c_double()
     _check(L.m3lb_stream_probe(int(nbytes), 1 if nontemporal else 0, int(reps), ctypes.byref(ms)))
     return ms.value
+
+
+def kernel_build_id():
+    """m3lb_kernel_build_id: the hash of the sources the loaded library was built from."""
+    return load().m3lb_kernel_build_id().decode()
 
 
 def tile_order_map(grid, run):
@@ -347,6 +357,21 @@ class FluidSolver:
 
     def calibrate_scales(self):
         _check(self.L.m3lb_calibrate_scales(self.h))
+
+    def set_step_fusion(self, mode, stripes=0, lag=0, blocks_per_cu=0, group=0):
+        """m3lb_set_step_fusion: pair steps in one pass (bit-identical results)."""
+        _check(self.L.m3lb_set_step_fusion(self.h, int(mode), int(stripes), int(lag),
+                                           int(blocks_per_cu), int(group)))
+
+    def fused_passes(self):
+        """m3lb_fused_passes: pairs of steps run as one pass so far."""
+        n = ctypes.c_uint64()
+        _check(self.L.m3lb_fused_passes(self.h, ctypes.byref(n)))
+        return n.value
+
+    def test_fail_fused_waits(self, on):
+        """m3lb_test_fail_fused_waits: failure-path test hook."""
+        _check(self.L.m3lb_test_fail_fused_waits(self.h, 1 if on else 0))
 
     def set_tile_order(self, run):
         _check(self.L.m3lb_set_tile_order(self.h, int(run)))

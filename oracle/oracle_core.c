@@ -33,7 +33,7 @@ typedef float REAL;
 #define R(x) ((REAL)(x))
 
 /* ---------------------------------------------------------------------------
- * Lattices, src/lattice/Lattices.hpp:130-214 (D2Q9, D3Q27).
+ * Lattices, src/lattice/Lattices.hpp:72-96, 125-159 (D2Q9, D3Q27).
  * ------------------------------------------------------------------------- */
 static const int CX9[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
 static const int CY9[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
@@ -65,18 +65,18 @@ static Lat lat_of(int lattice) {
   return l;
 }
 
-/* cvel<L>(i, a), Lattices.hpp:234-240 */
+/* cvel<L>(i, a), Lattices.hpp:178-185 */
 static inline int cvel(const Lat* l, int i, int a) {
   return a == 0 ? l->cx[i] : (a == 1 ? l->cy[i] : l->cz[i]);
 }
-/* weight<L,T>(i), Lattices.hpp:221-224 */
+/* weight<L,T>(i), Lattices.hpp:166-169 */
 static inline REAL weight(const Lat* l, int i) { return R(l->wn[i]) / R(l->wden); }
-/* cs2 = 1/3 and inv_cs2 = 3 on both lattices, Lattices.hpp:225-232 */
+/* cs2 = 1/3 and inv_cs2 = 3 on both lattices, Lattices.hpp:170-177 */
 #define CS2 (R(1) / R(3))
 #define ICS2 (R(3) / R(1))
 
 /* ---------------------------------------------------------------------------
- * Collision policy state (BGK.hpp:181-182, TRT.hpp:46-48,
+ * Collision policy state (BGK.hpp:43-44, TRT.hpp:46-48,
  * MomentCollision.hpp:89-91, Forcing.hpp:58-61).
  * ------------------------------------------------------------------------- */
 typedef struct {
@@ -163,16 +163,16 @@ static inline REAL guo_source_raw(const Lat* l, int i, const REAL F[3], REAL ux,
 
 typedef struct {
   REAL dens, ux, uy, uz;
-} Macro; /* BGK.hpp:164-167 */
+} Macro; /* BGK.hpp:26-29 */
 
-/* Collision::density, BGK.hpp:191-194 (NullCollision: m.dens,
+/* Collision::density, BGK.hpp:53-56 (NullCollision: m.dens,
  * tests/test_streaming.cpp:36) */
 static inline REAL density(const Coll* c, const Macro* m) {
   if (c->op == M3LB_OP_NULL) return m->dens;
   return c->shifted ? R(1) + m->dens : m->dens;
 }
 
-/* BGK/TRT/MomentCollision::macroscopic (BGK.hpp:201-216, TRT.hpp:69-84,
+/* BGK/TRT/MomentCollision::macroscopic (BGK.hpp:63-78, TRT.hpp:69-84,
  * MomentCollision.hpp:108-123, identical bodies) with Guo::shift_velocity
  * (Forcing.hpp:65-69).  NullCollision returns {1,0,0,0}
  * (tests/test_streaming.cpp:32-34). */
@@ -202,7 +202,7 @@ static inline Macro macroscopic(const Coll* c, const REAL* f) {
   return m;
 }
 
-/* BGK::collide, BGK.hpp:219-230 with Guo::source (Forcing.hpp:76-80) */
+/* BGK::collide, BGK.hpp:81-92 with Guo::source (Forcing.hpp:76-80) */
 static void collide_bgk(const Coll* c, int product, REAL* f, const Macro* m) {
   const Lat* l = &c->l;
   for (int i = 0; i < l->Q; ++i) {
@@ -214,13 +214,15 @@ static void collide_bgk(const Coll* c, int product, REAL* f, const Macro* m) {
   }
 }
 
-/* TRT::collide, TRT.hpp:92-127 (SecondOrderEquilibrium) */
-static void collide_trt(const Coll* c, REAL* f, const Macro* m) {
+/* TRT::collide, TRT.hpp:92-127, with the second-order (product = 0) or the
+ * product-form (product = 1) equilibrium: TRT<L, Eq, F, S> takes either
+ * (TRT.hpp:32-33, eq_of :86-90). */
+static void collide_trt(const Coll* c, int product, REAL* f, const Macro* m) {
   const Lat* l = &c->l;
   REAL e[27], sr[27];
   for (int i = 0; i < l->Q; ++i)
-    e[i] = c->shifted ? eq_shifted(l, 0, i, m->dens, m->ux, m->uy, m->uz)
-                      : eq(l, 0, i, m->dens, m->ux, m->uy, m->uz);
+    e[i] = c->shifted ? eq_shifted(l, product, i, m->dens, m->ux, m->uy, m->uz)
+                      : eq(l, product, i, m->dens, m->ux, m->uy, m->uz);
   if (c->forcing)
     for (int i = 0; i < l->Q; ++i) sr[i] = guo_source_raw(l, i, c->F, m->ux, m->uy, m->uz);
   const REAL op = c->omega, om = c->omega_m;
@@ -246,24 +248,24 @@ static void collide_trt(const Coll* c, REAL* f, const Macro* m) {
 }
 
 /* ---------------------------------------------------------------------------
- * ProductBasis<L>, src/collision/ProductBasis.hpp:191-342.
+ * ProductBasis<L>, src/collision/ProductBasis.hpp:55-206.
  * ------------------------------------------------------------------------- */
-/* dir_index<L>(a,b,c), ProductBasis.hpp:171-176 */
+/* dir_index<L>(a,b,c), ProductBasis.hpp:34-40 */
 static int dir_index(const Lat* l, int a, int b, int c) {
   for (int i = 0; i < l->Q; ++i)
     if (cvel(l, i, 0) == a && cvel(l, i, 1) == b && cvel(l, i, 2) == c) return i;
   return -1;
 }
-/* pi(a,b,c) table, ProductBasis.hpp:254-267 */
+/* pi(a,b,c) table, ProductBasis.hpp:118-131 */
 static int pi_of(const Lat* l, int a, int b, int c) {
   return dir_index(l, a - 1, b - 1, l->D == 3 ? c - 1 : 0);
 }
-/* mi(p,q,r), ProductBasis.hpp:198-200 */
+/* mi(p,q,r), ProductBasis.hpp:61-64 */
 static inline int mi(int D, int p, int q, int r) {
   return D == 2 ? (p * 3 + q) : ((p * 3 + q) * 3 + r);
 }
 
-/* fwd1d / inv1d, ProductBasis.hpp:273-292 */
+/* fwd1d / inv1d, ProductBasis.hpp:137-156 */
 static inline void fwd1d(REAL* a, REAL* b, REAL* c, REAL u) {
   const REAL cs2v = CS2;
   const REAL s0 = *a + *b + *c;
@@ -297,7 +299,7 @@ static PiTab pitab_of(const Lat* l) {
   return t;
 }
 
-/* to_moments, ProductBasis.hpp:295-317 */
+/* to_moments, ProductBasis.hpp:159-181 */
 static void to_moments(const Lat* l, const PiTab* t, const REAL* f, const REAL ub[3],
                        REAL* k) {
   const int D = l->D;
@@ -320,7 +322,7 @@ static void to_moments(const Lat* l, const PiTab* t, const REAL* f, const REAL u
   }
 }
 
-/* to_populations, ProductBasis.hpp:319-341 */
+/* to_populations, ProductBasis.hpp:183-205 */
 static void to_populations(const Lat* l, const PiTab* t, REAL* k, const REAL ub[3],
                            REAL* f) {
   const int D = l->D;
@@ -351,7 +353,7 @@ typedef struct {
 } EqF;
 
 /* eq_factors / ProductBasis::eq_1d, MomentCollision.hpp:128-132,
- * ProductBasis.hpp:243-247 */
+ * ProductBasis.hpp:107-111 */
 static EqF eq_factors(const REAL du[3], const REAL ub[3]) {
   EqF e;
   for (int a = 0; a < 3; ++a) {
@@ -440,13 +442,14 @@ static void collide(const Coll* c, const PiTab* t, REAL* f, const Macro* m) {
     case M3LB_OP_NULL: return;
     case M3LB_OP_BGK: collide_bgk(c, 0, f, m); return;
     case M3LB_OP_BGK_PRODUCT: collide_bgk(c, 1, f, m); return;
-    case M3LB_OP_TRT: collide_trt(c, f, m); return;
+    case M3LB_OP_TRT: collide_trt(c, 0, f, m); return;
+    case M3LB_OP_TRT_PRODUCT: collide_trt(c, 1, f, m); return;
     case M3LB_OP_MRT: collide_moment(c, t, 0, f, m); return;
     default: collide_moment(c, t, 1, f, m); return;
   }
 }
 
-/* seed_value for every operator: BGK.hpp:233-237, TRT.hpp:129-133,
+/* seed_value for every operator: BGK.hpp:95-99, TRT.hpp:129-133,
  * MomentCollision.hpp:320-330 (all Q values at once),
  * NullCollision tests/test_streaming.cpp:37. */
 static void seed_all(const Coll* c, const PiTab* t, REAL rho, REAL ux, REAL uy, REAL uz,
@@ -466,7 +469,7 @@ static void seed_all(const Coll* c, const PiTab* t, REAL rho, REAL ux, REAL uy, 
     to_populations(l, t, k, ub, out);
     return;
   }
-  const int product = c->op == M3LB_OP_BGK_PRODUCT;
+  const int product = c->op == M3LB_OP_BGK_PRODUCT || c->op == M3LB_OP_TRT_PRODUCT;
   for (int i = 0; i < l->Q; ++i)
     out[i] = c->shifted ? eq_shifted(l, product, i, rho - R(1), ux, uy, uz)
                         : eq(l, product, i, rho, ux, uy, uz);
@@ -594,13 +597,13 @@ static inline const Coll* coll_at(const S* s, int64_t n, Coll* tmp) {
 
 #define F(s, n, q) ((s)->f[(n) * (int64_t)(s)->Q + (q)])
 
-/* EsotericPull::Access<P>::load, EsotericPull.hpp:191-198 */
+/* EsotericPull::Access<P>::load, EsotericPull.hpp:65-72 */
 static inline REAL ep_load(const S* s, int P, const int64_t* j, int64_t n, int i) {
   if (i == 0) return F(s, n, 0);
   if (P == 0) return (i & 1) ? F(s, n, i) : F(s, j[i - 1], i);
   return (i & 1) ? F(s, n, i + 1) : F(s, j[i - 1], i - 1);
 }
-/* EsotericPull::Access<P>::scatter, EsotericPull.hpp:230-237 */
+/* EsotericPull::Access<P>::scatter, EsotericPull.hpp:104-111 */
 static inline void ep_scatter(S* s, int P, const int64_t* j, int64_t n, int i, REAL v) {
   if (i == 0) { F(s, n, 0) = v; return; }
   if (P == 0) {
@@ -1209,7 +1212,7 @@ static void* step_range(void* arg) {
       }
       if (flag == M3LB_SOLID) continue; /* implicit bounce-back, :759-761 */
       fill_neighbours(&s->d, &s->c.l, 1, 2, n, j);
-      /* load_rest / load_pair, EsotericPull.hpp:213-223 */
+      /* load_rest / load_pair, EsotericPull.hpp:87-97 */
       f[0] = F(s, n, 0);
       for (int i = 1; i < Q; i += 2) {
         if (P == 0) { f[i] = F(s, n, i); f[i + 1] = F(s, j[i], i + 1); }

@@ -93,12 +93,16 @@ int main() {
   };
   using BGKf = BGK<L, SecondOrderEquilibrium<L>, Guo, ShiftedPopulations>;
   using TRTf = TRT<L, SecondOrderEquilibrium<L>, Guo, ShiftedPopulations>;
+  // TRT takes any equilibrium (TRT.hpp:32-33); with u_y = 0 the D2Q9 product
+  // form reduces to the second-order one, so it must give TRT's slip row.
+  using TRTp = TRT<L, HighOrderEquilibrium<L>, Guo, ShiftedPopulations>;
   using MRTf = MRT<L, Guo, ShiftedPopulations>;
   using CMf = CentralMoments<L, Guo, ShiftedPopulations>;
   const double tb = 0.5 + std::sqrt(3.0) / 4.0;
   for (double tau : {0.6, 0.875, tb}) {
     run<BGKf>("BGK", tau, 32, plain);
     run<TRTf>("TRT", tau, 32, trt);
+    run<TRTp>("TRT_PRODUCT", tau, 32, trt);
     run<MRTf>("MRT", tau, 32, plain);
     run<CMf>("CM", tau, 32, plain);
   }

@@ -144,10 +144,49 @@ static void admission_checks() {
   }
 }
 
+// A seeding failure during admission: the failing box and every box admitted
+// after it are handed back unseeded, in order, and the error comes back; the
+// boxes before it stay.  With an arena behind it, the memory of the handed-back
+// boxes is free again.
+static void seed_failure_checks() {
+  m3lb::Arena a(1 << 20);
+  std::vector<int64_t> ids = {11, 12, 13, 14, 15};
+  std::map<int64_t, uint64_t> off;
+  for (int64_t id : ids) off[id] = a.alloc(4096);
+  std::vector<int64_t> seeded, undone;
+  const int rc = m3lb::seed_admitted(
+      ids,
+      [&](int64_t id) {
+        if (id == 13) return -3;  // the injected failure
+        seeded.push_back(id);
+        return 0;
+      },
+      [&](int64_t id) {
+        undone.push_back(id);
+        a.release(off[id], 4096);
+      });
+  CHECK(rc == -3);
+  CHECK((seeded == std::vector<int64_t>{11, 12}));
+  CHECK((undone == std::vector<int64_t>{13, 14, 15}));
+  CHECK(a.used() == 2 * 4096);
+  // no failure: every box seeded, nothing undone
+  seeded.clear();
+  undone.clear();
+  CHECK(m3lb::seed_admitted(ids, [&](int64_t id) { seeded.push_back(id); return 0; },
+                            [&](int64_t id) { undone.push_back(id); }) == 0);
+  CHECK(seeded == ids && undone.empty());
+  // the first box fails: all of them are handed back
+  undone.clear();
+  CHECK(m3lb::seed_admitted(ids, [&](int64_t) { return -1; },
+                            [&](int64_t id) { undone.push_back(id); }) == -1);
+  CHECK(undone == ids);
+}
+
 int main() {
   fastdiv_checks();
   arena_checks();
   admission_checks();
+  seed_failure_checks();
   if (failures) return 1;
   std::printf("ok\n");
   return 0;

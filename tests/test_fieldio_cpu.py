@@ -77,3 +77,19 @@ def test_pvd(tmp_path):
     fieldio.write_pvd(p, [(0.0, "f_0000.vti"), (50.0, "f_0001.vti")])
     t = open(p).read()
     assert '<DataSet timestep="50" group="" part="0" file="f_0001.vti"/>' in t
+
+
+def test_dat_per_column_formats(tmp_path):
+    """The campaign's tables print t* as %.6f and the ratios as %.8e
+    (validation/tgv3d.cpp:79-92): the reference's own first rows come back
+    character for character."""
+    g = GOLD["tgv3d_re1600_d3q27_cm"]
+    p = tmp_path / "t.dat"
+    fieldio.write_dat(p, g["header"][2:], g["rows"][:2], fmt=("%.6f", "%.8e", "%.8e"))
+    lines = open(p).read().splitlines()
+    assert lines[0] == g["header"]
+    assert lines[1] == "0.000000 1.00000000e+00 1.00000000e+00"
+    assert lines[2] == "0.500000 9.04202660e-01 1.61165631e+00"
+    import pytest
+    with pytest.raises(ValueError):
+        fieldio.write_dat(p, "x", [[1.0, 2.0]], fmt=("%.6f",))

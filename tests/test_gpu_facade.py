@@ -1,7 +1,8 @@
 """The C++ facade (include/m3lb/lbm.hpp) on the device: the Poiseuille program
 of tests/cpp/poiseuille_facade.cpp, written as the reference's
 validation/poiseuille.cpp drives FluidSolver, reproduces the reference's
-README slip table (validation/poiseuille.cpp:203-218)."""
+README slip table (validation/poiseuille.cpp:203-218); TRT with the
+product-form equilibrium (TRT<L, HighOrderEquilibrium<L>, ...>) gives TRT's row."""
 import os
 import subprocess
 import sys
@@ -28,17 +29,17 @@ def test_facade_poiseuille_matches_reference_table():
     g = GOLD["poiseuille_slip"]
     for row in g["rows"]:
         tau = 0.5 + 3 ** 0.5 / 4 if abs(row[0] - 0.9330127) < 1e-7 else row[0]
-        for col in ("BGK", "TRT", "MRT", "CM"):
+        for col in ("BGK", "TRT", "TRT_PRODUCT", "MRT", "CM"):
             key = (col, round(tau, 6))
             if key not in got:
                 continue
-            want = row[g["columns"].index(col)]
+            want = row[g["columns"].index("TRT" if col == "TRT_PRODUCT" else col)]
             _, slip = got[key]
             if abs(want) < 1e-11:
                 assert abs(slip) < 1e-11, (key, slip)
             else:
                 assert abs(slip - want) <= 0.0051 * abs(want), (key, slip, want)
-    assert len(got) == 12
+    assert len(got) == 15
 
 
 def test_facade_per_node_force_policy():

@@ -147,7 +147,7 @@ def wall_case(lat, dtype, op, storage, force):
     return cfg, fl
 
 
-OPS = [abi.OP_BGK, abi.OP_BGK_PRODUCT, abi.OP_TRT, abi.OP_MRT, abi.OP_CM]
+OPS = [abi.OP_BGK, abi.OP_BGK_PRODUCT, abi.OP_TRT, abi.OP_TRT_PRODUCT, abi.OP_MRT, abi.OP_CM]
 
 
 @pytest.mark.parametrize("force", [False, True])

@@ -138,3 +138,47 @@ def test_rescale_reencodes_exactly(dtype):
     assert np.array_equal(sa, sb)
     assert np.array_equal(sta, stb)
     assert np.array_equal(a, b)
+
+
+# Largest rounding of a value re-encoded at half its scale, in units of the
+# new step 1/s_q: I16 codes have step 1, E2M13 codes below 32 step 2^-8 (codes
+# that stay at or above 32 keep their 13 mantissa bits), FP16 subnormals step
+# 2^-24, BF16 subnormals 2^-133; normal floating codes are exact.
+HALF_STEP = {abi.I16: 0.5, abi.E2M13: 2.0 ** -9, abi.F16: 2.0 ** -25, abi.BF16: 2.0 ** -134}
+
+
+@pytest.mark.parametrize("dtype", [abi.I16, abi.E2M13, abi.F16, abi.BF16])
+def test_rescale_down_rounds_within_half_a_step(dtype):
+    """Halving every scale re-encodes each population to the nearest code of
+    the new scale: the decoded value moves by at most half a step of the new
+    code grid, and the forms with fixed-step codes do move."""
+    n = 16
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dtype, (n, n, n), omega=1.6)
+    with FluidSolver(cfg) as s:
+        s.initialize_field(*cases.abc_flow(cfg, 0.03))
+        s.step(10)
+        before = s.storage()
+        sc = s.scales()
+        s.set_scales(sc / 2.0)
+        after = s.storage()
+    bound = HALF_STEP[dtype] / (sc / 2.0)
+    dev = np.abs(after - before)
+    assert (dev <= bound[None, :] * (1 + 1e-12)).all(), (dev / bound[None, :]).max()
+    if dtype in (abi.I16, abi.E2M13):
+        assert dev.max() > 0.0
+
+
+@pytest.mark.parametrize("dtype", [abi.E2M13, abi.F16])
+def test_scales_outside_the_exact_range_are_refused(dtype):
+    """m3lb_set_scales accepts powers of two in 2^-60 .. 2^60, the range the
+    codecs' folded constants are exact over, and refuses the rest."""
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dtype, (8, 8, 8), omega=1.6)
+    with FluidSolver(cfg) as s:
+        s.initialize(1.0, 0.01, 0.0, 0.0)
+        ok = np.full(27, 2.0 ** 60)
+        s.set_scales(ok)
+        assert np.array_equal(s.scales(), ok)
+        for bad in (2.0 ** 61, 2.0 ** -61, 3.0, -4.0):
+            with pytest.raises(RuntimeError):
+                s.set_scales(np.full(27, bad))
+        assert np.array_equal(s.scales(), ok)

@@ -64,7 +64,7 @@ def _pair(cfg, walls, fd=True, flags=None):
     return s, o
 
 
-OPS = [abi.OP_BGK, abi.OP_BGK_PRODUCT, abi.OP_TRT, abi.OP_MRT, abi.OP_CM]
+OPS = [abi.OP_BGK, abi.OP_BGK_PRODUCT, abi.OP_TRT, abi.OP_TRT_PRODUCT, abi.OP_MRT, abi.OP_CM]
 
 
 @pytest.mark.parametrize("storage", [abi.SHIFTED, abi.RAW])

@@ -63,7 +63,7 @@ def magic_minus(w):
 # ---------------------------------------------------------------- lattices ---
 @pytest.mark.parametrize("lat", LATS)
 def test_lattice_identities(lat):
-    """src/lattice/Lattices.hpp:255-323 and tests/test_lattice.cpp:12-43, in
+    """src/lattice/Lattices.hpp:188-286 and tests/test_lattice.cpp:12-43, in
     exact integer arithmetic."""
     c = lattice.velocities(lat)
     wn, wd = lattice.weights_rational(lat)
@@ -162,7 +162,31 @@ def test_moment_seed_is_maxwellian_and_product_form(lat, storage):
 
 
 # ------------------------------------------------------------- collisions ---
-OPS = [abi.OP_BGK, abi.OP_BGK_PRODUCT, abi.OP_TRT, abi.OP_MRT, abi.OP_CM]
+OPS = [abi.OP_BGK, abi.OP_BGK_PRODUCT, abi.OP_TRT, abi.OP_TRT_PRODUCT, abi.OP_MRT, abi.OP_CM]
+
+
+@pytest.mark.parametrize("storage", [abi.RAW, abi.SHIFTED])
+@pytest.mark.parametrize("force", [None, FORCE])
+@pytest.mark.parametrize("lat", LATS)
+@pytest.mark.parametrize("eq", [(abi.OP_TRT, abi.OP_BGK), (abi.OP_TRT_PRODUCT, abi.OP_BGK_PRODUCT)])
+def test_trt_with_equal_rates_is_bgk(eq, lat, force, storage):
+    """TRT.hpp:11-17: BGK is TRT with omega_minus == omega_plus, for either
+    equilibrium TRT is given (TRT.hpp:32-33).  The two equilibria differ in
+    every direction (the product form carries the cubic and quartic terms),
+    so a TRT that ignored its equilibrium parameter fails one of the pairs."""
+    trt, bgk = eq
+    w = 1.37
+    ct = cfg_of(lat, trt, storage, omega=w, omega_minus=w, force=force)
+    cb = cfg_of(lat, bgk, storage, omega=w, force=force)
+    f = fill(lat, storage == abi.SHIFTED)
+    ot, mt = orc.collide_node(ct, f)
+    ob, mb = orc.collide_node(cb, f)
+    assert np.abs(mt - mb).max() == 0.0
+    assert np.abs(ot - ob).max() < 2e-16
+    # and the two equilibria really do lead to different post-collision states
+    other = orc.collide_node(cfg_of(lat, abi.OP_TRT_PRODUCT if trt == abi.OP_TRT else abi.OP_TRT,
+                                    storage, omega=w, omega_minus=w, force=force), f)[0]
+    assert np.abs(other - ot).max() > 1e-9
 
 
 @pytest.mark.parametrize("dtype", [abi.F64, abi.F32])

@@ -22,7 +22,7 @@ from m3lb_amd import abi, cases  # noqa: E402
 from oracle.binding import OracleSolver  # noqa: E402
 from golden_cases import interior2d  # noqa: E402
 
-OPS = [abi.OP_BGK, abi.OP_TRT, abi.OP_MRT, abi.OP_CM]
+OPS = [abi.OP_BGK, abi.OP_TRT, abi.OP_TRT_PRODUCT, abi.OP_MRT, abi.OP_CM]
 
 
 def _channel(lat, op, storage, omega, U, F, ny=9, steps=None):

@@ -27,7 +27,7 @@ def _channel():
     shape = (24, 20, 16)
     cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, abi.F64, shape, periodic=(1, 0, 1),
                           omega=1.3, force=(1e-5, 0.0, 0.0))
-    fl = cases.channel_flags(shape)
+    fl = cases.channel_flags(shape).reshape(shape[::-1])
     z, y, x = np.meshgrid(*(np.arange(n) for n in shape[::-1]), indexing="ij")
     fl[(x - 8) ** 2 + (y - 9) ** 2 + (z - 7) ** 2 <= 9] = abi.SOLID
     return cfg, fl

@@ -339,6 +339,15 @@ int m3lb_set_cache_policy(m3lb_solver* s, int32_t policy);
 enum { M3LB_FUSION_AUTO = 0, M3LB_FUSION_OFF = 1, M3LB_FUSION_ON = 2 };
 int m3lb_set_step_fusion(m3lb_solver* s, int32_t mode, int32_t stripes, int32_t lag,
                          int32_t blocks_per_cu, int32_t group);
+/* total_mass and total_momentum(0..2) at once, out = {mass, Mx, My, Mz},
+ * bit-identical to those two calls, without allocating: the per-block
+ * partials live in a static device array of 4096 * 4 + 64 doubles (131,584
+ * bytes), one per GPU, kept until the process ends.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving
+ * loops, code that forbids hipMalloc).  Calls that overlap on one GPU share
+ * that array and race; one call at a time per GPU. */
+int m3lb_totals_static(m3lb_solver* s, double out[4]);
+
 /* Pairs of steps the handle has run as one pass since creation. */
 int m3lb_fused_passes(const m3lb_solver* s, uint64_t* passes);
 /* Failure-path test hook: with on != 0 every wait of a two-step pass is made

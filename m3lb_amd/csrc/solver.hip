@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1019,6 +1020,80 @@ static int reduce4(m3lb_solver* s, double out[4]) {
   for (int k = 0; k < 4; ++k) out[k] = 0;
   for (unsigned b = 0; b < grid; ++b)
     for (int k = 0; k < 4; ++k) out[k] += h[size_t(b) * 4 + k];
+  return M3LB_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Totals without allocation (m3lb_totals_static).  The per-block partials and
+// the folded result live in a file-scope device array of fixed size, one copy
+// per GPU, addressed through the pointer hipGetSymbolAddress returns for the
+// device (the array's own name is a host-side stand-in, not a device address).
+// Every call on a GPU uses that GPU's copy: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr unsigned kScratchBlocks = 4096;  // = the largest reduce_grid
+constexpr size_t kScratchDoubles = size_t(kScratchBlocks) * 4 + 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Sums each of the 4 columns of `grid` rows of partials in row order into
+// out[0..3]: the order, and so the bits, of reduce4's host loop.
+__global__ void k_fold_rows4(const double* partial, unsigned grid, double* out) {
+  const unsigned k = threadIdx.x;
+  if (k >= 4) return;
+  double s = 0;
+  for (unsigned b = 0; b < grid; ++b) s += partial[size_t(b) * 4 + k];
+  out[k] = s;
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's copy of `scratch`, looked up once per device with it current.
+int scratch_of(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(M3LB_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    HIP_OK(hipGetDevice(&cur));
+    if (cur != device) HIP_OK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) HIP_OK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(M3LB_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return M3LB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int m3lb_totals_static(m3lb_solver* s, double out[4]) {
+  if (!s || !out) return fail(M3LB_EINVAL, "null argument");
+  const unsigned grid = reduce_grid(s);
+  if (size_t(grid) * 4 + 4 > kScratchDoubles)
+    return fail(M3LB_EINVAL, "totals_static: " + std::to_string(grid) +
+                                 " reduction blocks exceed the static scratch of " +
+                                 std::to_string(kScratchBlocks));
+  double* sc = nullptr;
+  int rc = scratch_of(s->device, &sc);
+  if (rc) return rc;
+  double* result = sc + size_t(kScratchBlocks) * 4;
+  m3lb::ReadArgs a = read_args(s);
+  a.out = sc;  // every block writes its 4 partials: nothing to clear
+  s->ks->reduce_mass(a, grid, s->stream);
+  rc = launch_ok();
+  if (rc) return rc;
+  k_fold_rows4<<<1, 64, 0, s->stream>>>(sc, grid, result);
+  rc = launch_ok();
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(out, result, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  SYNC_OK(s);
   return M3LB_OK;
 }
 

@@ -71,6 +71,7 @@ SIGNATURES = {
     "m3lb_set_step_fusion": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int32, ctypes.c_int32]),
     "m3lb_fused_passes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "m3lb_totals_static": (ctypes.c_int, [_vp, _dp]),
     "m3lb_test_fail_fused_waits": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "m3lb_initialize_taylor_green": (ctypes.c_int, [_vp, ctypes.c_double]),
     "m3lb_momentum_exchange": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8), _dp]),
@@ -362,6 +363,12 @@ class FluidSolver:
         """m3lb_set_step_fusion: pair steps in one pass (bit-identical results)."""
         _check(self.L.m3lb_set_step_fusion(self.h, int(mode), int(stripes), int(lag),
                                            int(blocks_per_cu), int(group)))
+
+    def totals_static(self):
+        """m3lb_totals_static: (mass, Mx, My, Mz) with no device allocation."""
+        out = np.zeros(4)
+        _check(self.L.m3lb_totals_static(self.h, _d(out)))
+        return out
 
     def fused_passes(self):
         """m3lb_fused_passes: pairs of steps run as one pass so far."""

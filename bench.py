@@ -3,7 +3,10 @@
 Workload: D3Q27 factorised central moments, FP32 shifted populations, 512^3
 periodic Taylor-Green vortex (u0 = 0.04, Re = 1600 on L = D/2pi, seeded uniform
 perturbation 1e-4), the configuration the throughput metric is quoted on.  A
-step is one fused stream+collide launch over the whole box.
+step is one stream+collide update of every node; the library runs the steps of
+this box in pairs, one pass over the population array per pair (a persistent
+launch and a short launch for the rows it defers), and the rate is steps times
+nodes over the time of the whole sequence.
 
 Prints ONE JSON line on stdout: MLUPS over interior nodes (the reference's
 convention, src/app/main.cpp:50), the HBM roofline of the step kernel measured
@@ -472,11 +475,14 @@ def main():
         f"nontemporal {ceil['nontemporal']:.0f} GB/s; step kernel at "
         f"{r['achieved_gbs'] / best_ceiling:.3f} of the better")
 
-    # HBM bytes per launch from the committed rocprofv3 --pmc summary of this
-    # exact workload (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel),
+    # Bytes per step between the L2 caches and the memory fabric, from the
+    # committed rocprofv3 --pmc summary of this exact workload (FETCH_SIZE +
+    # WRITE_SIZE of one two-step pass, calibrated on a copy kernel, halved),
     # used only while the loaded library holds the kernels it was measured on.
+    # These counters see requests that the memory-side 256 MB cache serves as
+    # well as those that reach HBM.
     traffic, traffic_src = None, None
-    tj = os.path.join(ROOT, "profiles", "k_step_cm_f32_512cubed_traffic.json")
+    tj = os.path.join(ROOT, "profiles", "k_pair_cm_f32_512cubed_traffic.json")
     build = solver.kernel_build_id()
     if args.size == 512 and args.dtype == "f32" and os.path.exists(tj):
         with open(tj) as fh:

@@ -393,9 +393,11 @@ int m3lb_initialize_taylor_green(m3lb_solver* s, double u0);
  * budget and advances all of them together: each step is one persistent
  * kernel over the packed tile list of every resident box; a chunk of
  * steps_per_chunk steps is one hipGraph launch.  Requests are admitted first
- * come, first served as slots and memory free up (a bounded number of smaller
- * requests may pass a queue head that does not fit yet), and complete
- * independently.  A box computes bit-identical results alone, in any mix, and
+ * come, first served as slots and memory free up, and complete
+ * independently.  Behind a queue head that does not fit yet, a later request
+ * is admitted only when its last step comes no later than the step after
+ * which the head fits (every resident box advances one step per step), so
+ * passing never delays the head; at most max_bypass pass one head.  A box computes bit-identical results alone, in any mix, and
  * in the single-box solver with the same configuration and forcing policy.
  * Every box of a server shares one kernel configuration (lattice, operator,
  * storage, dtype, forcing policy); F32 and F64 storage only.
@@ -417,7 +419,8 @@ typedef struct m3lb_server_config {
   int32_t tile_shape;      /* M3LB_SHAPE_*: ROWS = whole x-rows per tile (as the
                               single-box step), LINEAR = 256 consecutive
                               interior nodes, AUTO = ROWS when nx % 64 == 0 */
-  int32_t max_bypass;      /* admissions allowed past a waiting queue head */
+  int32_t max_bypass;      /* admissions allowed past one waiting queue head
+                              (each also bounded as above) */
   uint64_t budget_bytes;   /* device arena for resident boxes */
   uint64_t max_resident_nodes; /* admission stops while the resident boxes hold
                               this many interior nodes (one box is always

@@ -325,10 +325,12 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 // boxes' tile counts); a persistent grid of G blocks (G a multiple of 8) walks
 // it in virtual-tile order v = block + k*G, mapped through tile_of so that
 // runs of consecutive tiles stay on one XCD.  Step j of a chunk of steps is
-// launch j; box b takes part while j < steps, at parity (parity0 + j) & 1.  A
-// box's step barrier is the kernel boundary.  Box count and tile total are
-// read from device memory, so one captured sequence of launches serves every
-// chunk.  The per-node work is node_update, as in k_step, so a box gets
+// launch j; box b takes part while j < steps, at parity (parity0 + j) & 1.
+// The boxes are packed in order of descending steps, so launch j walks only
+// the prefix of the list that steps at j (the tiles of boxes that have
+// stopped are not visited).  A box's step barrier is the kernel boundary.
+// Box count and tile totals are read from device memory, so one captured
+// sequence of launches serves every chunk.  The per-node work is node_update, as in k_step, so a box gets
 // bit-identical results alone, in any mix of boxes, and in the single-box
 // solver.
 // ---------------------------------------------------------------------------
@@ -351,7 +353,10 @@ struct BoxDesc {
 
 struct BatchArgs {
   const BoxDesc* desc;
-  const unsigned* hdr;  // {boxes, total tiles, tile_start[0..boxes]}
+  const unsigned* hdr;  // {boxes, total tiles, tile_start[0..boxes], stepping[0..K-1]}:
+                        // the boxes of a chunk in order of descending steps, and
+                        // stepping[j] = tiles of the boxes that step at j, a
+                        // prefix of the packed list (serve_plan.hpp pack_chunk)
   int j;                // step within the chunk
   int run;              // tile order, see tile_of()
   int cached;           // 1: plain accesses everywhere (resident set in cache)
@@ -393,17 +398,18 @@ __device__ __forceinline__ void batched_tile(const BoxDesc& b, unsigned local, i
 
 // Tile order of the batched kernel: run >= -1 as tile_of (virtual tile
 // v = block + k * grid, one box search per tile); kRunBlockRanges: block b
-// takes the contiguous range [b * per, (b + 1) * per) of the packed list,
-// searches once and walks across box boundaries, skipping a box with no
-// step at j as a whole.
+// takes the contiguous range [b * per, (b + 1) * per) of the stepping prefix,
+// searches once and walks across box boundaries.
 constexpr int kRunBlockRanges = -2;
 
 // RANGES: the kRunBlockRanges order (a separate instance, so that neither
 // order's loop raises the other's register count).
 template <class K, bool RANGES, bool NTL>
 __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
-  const unsigned nbox = a.hdr[0], total = a.hdr[1];
-  const unsigned* start = a.hdr + 2;  // start[nbox] == total
+  const unsigned nbox = a.hdr[0];
+  const unsigned* start = a.hdr + 2;  // start[nbox] == all tiles of the chunk
+  // only the tiles of boxes that step at j: the list's first `total`
+  const unsigned total = a.hdr[3 + nbox + unsigned(a.j)];
   if constexpr (RANGES) {
     const unsigned per = (total + gridDim.x - 1) / gridDim.x;
     unsigned t = blockIdx.x * per;
@@ -419,9 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
       while (start[lo + 1] <= t) ++lo;
       const BoxDesc& b = a.desc[lo];
       const unsigned stop = min(end, start[lo + 1]);
-      if (a.j < b.steps)
-        for (; t < stop; ++t) batched_tile<K, NTL>(b, t - start[lo], a.j);
-      t = stop;
+      for (; t < stop; ++t) batched_tile<K, NTL>(b, t - start[lo], a.j);
     }
     return;
   }
@@ -433,9 +437,7 @@ __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
       if (start[mid] <= t) lo = mid;
       else hi = mid;
     }
-    const BoxDesc& b = a.desc[lo];
-    if (a.j >= b.steps) continue;
-    batched_tile<K, NTL>(b, t - start[lo], a.j);
+    batched_tile<K, NTL>(a.desc[lo], t - start[lo], a.j);
   }
 }
 

@@ -32,6 +32,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr int kDepth = 2;          // chunks in flight
+constexpr int kMaxChunk = 1024;    // steps per chunk and per autotune measurement
 constexpr unsigned kSumGrid = 1024;  // blocks of a completion reduction
 
 int err(int code, const std::string& m) { return m3lb::set_error(code, m.c_str()); }
@@ -149,7 +150,7 @@ int validate(const m3lb_server_config* c) {
   if (c->forcing != M3LB_FORCE_NONE && c->forcing != M3LB_FORCE_GUO)
     return err(M3LB_EINVAL, "forcing must be M3LB_FORCE_NONE or M3LB_FORCE_GUO");
   if (c->max_boxes < 1 || c->max_boxes > 65536) return err(M3LB_EINVAL, "max_boxes must be in 1..65536");
-  if (c->steps_per_chunk < 1 || c->steps_per_chunk > 1024)
+  if (c->steps_per_chunk < 1 || c->steps_per_chunk > kMaxChunk)
     return err(M3LB_EINVAL, "steps_per_chunk must be in 1..1024");
   if (c->blocks_per_cu < 1 || c->blocks_per_cu > 64) return err(M3LB_EINVAL, "blocks_per_cu must be in 1..64");
   if (c->tile_shape < M3LB_SHAPE_ROWS || c->tile_shape > M3LB_SHAPE_AUTO)
@@ -332,16 +333,44 @@ int admit(m3lb_server* s) {
   int rc = M3LB_OK;
   const uint64_t cap = s->cfg.max_resident_nodes;
   uint64_t nodes = s->resident_nodes;
-  std::vector<int64_t> ids = s->adm.run(s->free_slots.size(), [&](int64_t id) {
-    Box& b = s->boxes.at(id);
-    const uint64_t bn = uint64_t(b.d.nx) * uint64_t(b.d.ny) * uint64_t(b.d.nz);
-    if (cap && nodes > 0 && nodes + bn > cap) return false;
-    const uint64_t off = s->arena.alloc(b.bytes);
-    if (off == m3lb::Arena::kNone) return false;
-    b.off = off;
-    nodes += bn;
-    return true;
-  });
+  // the resident boxes, and those admitted in this pass, as admission sees them
+  std::vector<m3lb::Holding> hold;
+  uint64_t resident_bytes = 0;
+  for (int64_t id : s->resident) {
+    const Box& b = s->boxes.at(id);
+    hold.push_back({b.steps_left, uint64_t(b.d.nx) * uint64_t(b.d.ny) * uint64_t(b.d.nz), b.bytes});
+    resident_bytes += m3lb::Arena::round(b.bytes);
+  }
+  // arena bytes no resident box holds: kept results, which stepping never frees
+  const uint64_t fixed = s->arena.used() - resident_bytes;
+  int64_t head = -1, horizon = 0;
+  std::vector<int64_t> ids = s->adm.run(
+      s->free_slots.size(),
+      [&](int64_t id) {
+        Box& b = s->boxes.at(id);
+        const uint64_t bn = uint64_t(b.d.nx) * uint64_t(b.d.ny) * uint64_t(b.d.nz);
+        if (cap && nodes > 0 && nodes + bn > cap) return false;
+        const uint64_t off = s->arena.alloc(b.bytes);
+        if (off == m3lb::Arena::kNone) return false;
+        b.off = off;
+        nodes += bn;
+        hold.push_back({b.req.steps, bn, b.bytes});
+        return true;
+      },
+      // A request passes the waiting head only if its last step is issued no
+      // later than the step after which the head fits: every resident box
+      // advances one step per step, so it has left by then and the head is
+      // admitted no later than without it.
+      [&](int64_t waiting, int64_t id) {
+        if (waiting != head) {
+          head = waiting;
+          const Box& h = s->boxes.at(waiting);
+          horizon = m3lb::fit_horizon(hold, fixed, cap, s->arena.capacity(),
+                                      uint64_t(h.d.nx) * uint64_t(h.d.ny) * uint64_t(h.d.nz),
+                                      h.bytes);
+        }
+        return int64_t(s->boxes.at(id).req.steps) <= horizon;
+      });
   s->resident_nodes = nodes;
   const double t = now_s(s);
   for (int64_t id : ids) {
@@ -422,35 +451,39 @@ int issue(m3lb_server* s, bool* issued) {
   c.finishing.clear();
   c.lups = 0;
   const int K = s->cfg.steps_per_chunk;
-  unsigned n = 0, total = 0;
   int jmax = 0;
+  std::vector<unsigned> tiles;
   for (int64_t id : s->resident) {
     Box& b = s->boxes.at(id);
     if (b.steps_left <= 0) continue;
     const int k = std::min(K, b.steps_left);
-    BoxDesc& x = s->desc_h[buf][n];
-    x = b.desc;
-    x.parity0 = b.parity;
-    x.steps = k;
-    s->hdr_h[buf][2 + n] = total;
-    total += b.tiles;
-    ++n;
-    b.steps_left -= k;
-    b.parity = (b.parity + k) & 1;
     c.boxes.push_back(id);
     c.steps.push_back(k);
-    c.lups += uint64_t(b.desc.nodes) * uint64_t(k);
-    jmax = std::max(jmax, k);
-    if (b.steps_left == 0) c.finishing.push_back(id);
+    tiles.push_back(b.tiles);
   }
+  const size_t n = c.boxes.size();
   if (n == 0) return M3LB_OK;
-  s->hdr_h[buf][0] = n;
-  s->hdr_h[buf][1] = total;
-  s->hdr_h[buf][2 + n] = total;
+  // boxes that stop within the chunk go last, so that launch j walks only
+  // the tiles of the boxes that step at j
+  std::vector<size_t> order;
+  m3lb::pack_chunk(tiles, c.steps, K, order, s->hdr_h[buf]);
+  for (size_t k = 0; k < n; ++k) {
+    Box& b = s->boxes.at(c.boxes[order[k]]);
+    const int steps = c.steps[order[k]];
+    BoxDesc& x = s->desc_h[buf][k];
+    x = b.desc;
+    x.parity0 = b.parity;
+    x.steps = steps;
+    b.steps_left -= steps;
+    b.parity = (b.parity + steps) & 1;
+    c.lups += uint64_t(b.desc.nodes) * uint64_t(steps);
+    jmax = std::max(jmax, steps);
+    if (b.steps_left == 0) c.finishing.push_back(b.id);
+  }
   SRV_HIP(hipEventRecord(c.begin, s->stream));
-  SRV_HIP(hipMemcpyAsync(s->desc_d[buf], s->desc_h[buf], size_t(n) * sizeof(BoxDesc),
+  SRV_HIP(hipMemcpyAsync(s->desc_d[buf], s->desc_h[buf], n * sizeof(BoxDesc),
                          hipMemcpyHostToDevice, s->stream));
-  SRV_HIP(hipMemcpyAsync(s->hdr_d[buf], s->hdr_h[buf], size_t(n + 3) * sizeof(unsigned),
+  SRV_HIP(hipMemcpyAsync(s->hdr_d[buf], s->hdr_h[buf], m3lb::pack_words(n, K) * sizeof(unsigned),
                          hipMemcpyHostToDevice, s->stream));
   if (s->exec[buf] && jmax == K) {
     SRV_HIP(hipGraphLaunch(s->exec[buf], s->stream));
@@ -623,10 +656,10 @@ int m3lb_srv_create(const m3lb_server_config* cfg, m3lb_server** out) {
     if (hipEventCreate(&s->chunk[k].begin) != hipSuccess || hipEventCreate(&s->chunk[k].end) != hipSuccess)
       return bail(err(M3LB_EDEVICE, "hipEventCreate failed"));
     if (hipHostMalloc(reinterpret_cast<void**>(&s->desc_h[k]), nb * sizeof(BoxDesc), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&s->hdr_h[k]), (nb + 3) * sizeof(unsigned), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc(reinterpret_cast<void**>(&s->hdr_h[k]), m3lb::pack_words(nb, kMaxChunk) * sizeof(unsigned), hipHostMallocDefault) != hipSuccess)
       return bail(err(M3LB_ENOMEM, "pinned descriptor buffers"));
     if (hipMalloc(&s->desc_d[k], nb * sizeof(BoxDesc)) != hipSuccess ||
-        hipMalloc(&s->hdr_d[k], (nb + 3) * sizeof(unsigned)) != hipSuccess)
+        hipMalloc(&s->hdr_d[k], m3lb::pack_words(nb, kMaxChunk) * sizeof(unsigned)) != hipSuccess)
       return bail(err(M3LB_ENOMEM, "device descriptor buffers"));
   }
   if (hipMalloc(&s->partial_d, kSumGrid * 4 * sizeof(double)) != hipSuccess ||
@@ -858,7 +891,7 @@ int m3lb_srv_clock(m3lb_server* s, double* seconds) {
 
 int m3lb_srv_autotune(m3lb_server* s, int32_t steps, m3lb_tune_result* out) {
   if (!s || !out) return err(M3LB_EINVAL, "null argument");
-  if (steps < 1 || steps > 1024) return err(M3LB_EINVAL, "steps must be in 1..1024");
+  if (steps < 1 || steps > kMaxChunk) return err(M3LB_EINVAL, "steps must be in 1..1024");
   if (!idle(s)) return err(M3LB_ESTATE, "autotune needs an idle server");
   // The mix: Taylor-Green boxes, edges log-uniform on [32, 192] from a fixed
   // sequence, up to max_boxes / 256 boxes, half the budget and 8 GiB, and
@@ -918,23 +951,25 @@ int m3lb_srv_autotune(m3lb_server* s, int32_t steps, m3lb_tune_result* out) {
     s->grid = unsigned(s->cus * s->cfg.blocks_per_cu + 7) / 8 * 8;
     float ms = 0;
     for (int rep = 0; rep < 2; ++rep) {
-      unsigned total = 0, n = 0;
+      const size_t n = mix.size();
+      std::vector<unsigned> tiles;
       for (Box& b : mix) {
         describe(s, b);
-        BoxDesc& x = s->desc_h[0][n];
+        tiles.push_back(b.tiles);
+      }
+      std::vector<size_t> order;
+      m3lb::pack_chunk(tiles, std::vector<int>(n, steps), steps, order, s->hdr_h[0]);
+      for (size_t k = 0; k < n; ++k) {
+        Box& b = mix[order[k]];
+        BoxDesc& x = s->desc_h[0][k];
         x = b.desc;
         x.parity0 = b.parity;
         x.steps = steps;
-        s->hdr_h[0][2 + n] = total;
-        total += b.tiles;
-        ++n;
         b.parity = (b.parity + steps) & 1;
       }
-      s->hdr_h[0][0] = n;
-      s->hdr_h[0][1] = total;
-      s->hdr_h[0][2 + n] = total;
       SRV_HIP(hipMemcpyAsync(s->desc_d[0], s->desc_h[0], n * sizeof(BoxDesc), hipMemcpyHostToDevice, s->stream));
-      SRV_HIP(hipMemcpyAsync(s->hdr_d[0], s->hdr_h[0], (n + 3) * sizeof(unsigned), hipMemcpyHostToDevice, s->stream));
+      SRV_HIP(hipMemcpyAsync(s->hdr_d[0], s->hdr_h[0], m3lb::pack_words(n, steps) * sizeof(unsigned),
+                             hipMemcpyHostToDevice, s->stream));
       SRV_HIP(hipEventRecord(e0, s->stream));
       for (int j = 0; j < steps; ++j) s->ks->batched(batch_args(s, 0, j), s->grid, s->stream);
       int lrc = launch_ok();

@@ -3,16 +3,23 @@
 //   Arena      first-fit allocator over one device allocation (the HBM budget),
 //              256-byte granularity, freed ranges coalesce with their neighbours;
 //   Admission  the order in which queued requests are admitted: first come,
-//              first served, with a bounded number of smaller requests allowed
-//              past a queue head that does not fit yet (backfill);
+//              first served; a later request may pass a queue head that does
+//              not fit yet when it will have finished before the head can be
+//              admitted (backfill that never delays the head), at most
+//              max_bypass of them per head;
+//   fit_horizon  when that head can be admitted, counted in steps;
+//   pack_chunk  the packed tile list of one chunk, with the boxes that stop
+//              early at its end, so that step j covers a prefix of it;
 //   seed_admitted  seeds the boxes of one admission pass and, at the first
 //              failure, hands that box and every later one back unseeded.
 // The reference has no serving layer; nothing here has a counterpart in it.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <deque>
 #include <iterator>
+#include <limits>
 #include <map>
 #include <vector>
 
@@ -89,11 +96,55 @@ class Arena {
   uint64_t cap_ = 0, used_ = 0, peak_ = 0;
 };
 
+// A resident box as admission sees it: steps not yet issued, interior nodes
+// and arena bytes.
+struct Holding {
+  int64_t steps_left;
+  uint64_t nodes, bytes;
+};
+
+constexpr int64_t kNoHorizon = std::numeric_limits<int64_t>::max();
+
+// Every resident box advances one step per step of the server, and leaves
+// (returning its nodes and bytes) once its steps run out.  Returns the number
+// of steps after which a request of need_nodes nodes and need_bytes bytes
+// fits: under the node cap (0: none; a request always fits when no nodes are
+// resident) and within the budget, counting fixed_bytes held by no resident
+// box (kept results).  0 when it fits now; kNoHorizon when the resident boxes
+// leaving is not enough.  Bytes are counted in total, so an arena too
+// fragmented for the request can still make it wait past the horizon.
+inline int64_t fit_horizon(std::vector<Holding> res, uint64_t fixed_bytes, uint64_t node_cap,
+                           uint64_t budget, uint64_t need_nodes, uint64_t need_bytes) {
+  need_bytes = Arena::round(need_bytes);
+  uint64_t nodes = 0, bytes = fixed_bytes;
+  for (const Holding& h : res) {
+    nodes += h.nodes;
+    bytes += Arena::round(h.bytes);
+  }
+  auto fits = [&]() {
+    const bool node_ok = node_cap == 0 || nodes == 0 || nodes + need_nodes <= node_cap;
+    return node_ok && bytes + need_bytes <= budget;
+  };
+  if (fits()) return 0;
+  std::sort(res.begin(), res.end(),
+            [](const Holding& a, const Holding& b) { return a.steps_left < b.steps_left; });
+  for (size_t k = 0; k < res.size();) {
+    const int64_t s = res[k].steps_left;
+    for (; k < res.size() && res[k].steps_left == s; ++k) {
+      nodes -= res[k].nodes;
+      bytes -= Arena::round(res[k].bytes);
+    }
+    if (fits()) return s;
+  }
+  return kNoHorizon;
+}
+
 // Queue of request ids in arrival order.  run() walks it front to back and
 // calls try_admit(id) for each while slots are free; try_admit reserves what
 // the request needs and returns true, or returns false when it does not fit
-// now.  A queue head that does not fit lets at most max_bypass later requests
-// be admitted before it; after that nothing more is admitted until it fits.
+// now.  Behind a queue head that does not fit, a later request is tried only
+// when may_pass(head, id) allows it, and at most max_bypass are admitted
+// before the head; after that nothing more is admitted until it fits.
 class Admission {
  public:
   explicit Admission(int max_bypass = 64) : max_bypass_(max_bypass) {}
@@ -114,19 +165,25 @@ class Admission {
   int bypassed() const { return bypass_; }
   void set_max_bypass(int m) { max_bypass_ = m; }
 
-  template <class TryAdmit>
-  std::vector<int64_t> run(size_t slots_free, TryAdmit&& try_admit) {
+  template <class TryAdmit, class MayPass>
+  std::vector<int64_t> run(size_t slots_free, TryAdmit&& try_admit, MayPass&& may_pass) {
     std::vector<int64_t> admitted;
     auto it = q_.begin();
     bool head = true;  // `it` is the current queue head
+    int64_t waiting = -1;
     while (it != q_.end() && admitted.size() < slots_free) {
       if (!head && bypass_ >= max_bypass_) break;
+      if (!head && !may_pass(waiting, *it)) {
+        ++it;
+        continue;
+      }
       if (try_admit(*it)) {
         admitted.push_back(*it);
         if (head) bypass_ = 0;
         else ++bypass_;
         it = q_.erase(it);
       } else {
+        if (head) waiting = *it;
         ++it;
         head = false;
       }
@@ -134,11 +191,50 @@ class Admission {
     return admitted;
   }
 
+  // Any later request may pass (up to max_bypass).
+  template <class TryAdmit>
+  std::vector<int64_t> run(size_t slots_free, TryAdmit&& try_admit) {
+    return run(slots_free, try_admit, [](int64_t, int64_t) { return true; });
+  }
+
  private:
   std::deque<int64_t> q_;
   int max_bypass_;
   int bypass_ = 0;  // admissions past the current head since it became head
 };
+
+// The packed tile list of one chunk of K steps.  Box k of the input has
+// tiles[k] tiles and takes part in steps[k] (1..K) steps of the chunk.  The
+// boxes are laid out in order of descending steps (ties keep their input
+// order), so that the boxes still stepping at step j are a prefix of the list.
+// order receives the input index of each laid-out box; hdr receives
+//   hdr[0] = boxes, hdr[1] = total tiles, hdr[2 + k] = first tile of laid-out
+//   box k (k = 0..boxes, hdr[2 + boxes] = total), and
+//   hdr[3 + boxes + j] = tiles of the boxes stepping at step j (j = 0..K-1),
+// pack_words(boxes, K) words in all.
+inline size_t pack_words(size_t boxes, int K) { return boxes + 3 + size_t(K); }
+
+inline void pack_chunk(const std::vector<unsigned>& tiles, const std::vector<int>& steps, int K,
+                       std::vector<size_t>& order, unsigned* hdr) {
+  const size_t n = tiles.size();
+  order.resize(n);
+  for (size_t k = 0; k < n; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return steps[a] > steps[b]; });
+  unsigned total = 0;
+  for (size_t k = 0; k < n; ++k) {
+    hdr[2 + k] = total;
+    total += tiles[order[k]];
+  }
+  hdr[0] = unsigned(n);
+  hdr[1] = total;
+  hdr[2 + n] = total;
+  size_t m = n;  // boxes stepping at step j: the first m of the layout
+  for (int j = 0; j < K; ++j) {
+    while (m > 0 && steps[order[m - 1]] <= j) --m;
+    hdr[3 + n + size_t(j)] = hdr[2 + m];
+  }
+}
 
 // Seeds the boxes admitted in one pass, in admission order.  seed(id) returns
 // 0 or an error code.  At the first failure undo(id) is called for that box and

@@ -3,6 +3,7 @@
 // by tests/test_serve_cpu.py; prints "ok" and exits 0 when every check holds.
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <random>
 #include <set>
 #include <vector>
@@ -144,6 +145,161 @@ static void admission_checks() {
   }
 }
 
+// Backfill that never delays the head: fit_horizon against a step-by-step
+// replay, may_pass in Admission::run, and a lockstep simulation in which every
+// head is admitted no later than the horizon it had when it first waited.
+static void horizon_checks() {
+  using m3lb::Holding;
+  // fits now; node cap; ties leave together; bytes; empty server
+  CHECK(m3lb::fit_horizon({{5, 100, 256}}, 0, 1000, 1 << 20, 100, 256) == 0);
+  const std::vector<Holding> r = {{5, 100, 256}, {3, 200, 256}, {8, 50, 256}, {3, 10, 256}};
+  CHECK(m3lb::fit_horizon(r, 0, 300, 1 << 20, 150, 256) == 3);
+  CHECK(m3lb::fit_horizon(r, 0, 300, 1 << 20, 250, 256) == 5);
+  CHECK(m3lb::fit_horizon(r, 0, 300, 1 << 20, 10000, 256) == 8);  // empty server
+  CHECK(m3lb::fit_horizon(r, 0, 0, 4 * 256 + 256, 1, 512) == 3);   // bytes only
+  CHECK(m3lb::fit_horizon(r, 1 << 20, 0, 1 << 20, 1, 256) == m3lb::kNoHorizon);
+  // against a replay: step until the request fits
+  std::mt19937_64 rng(11);
+  for (int t = 0; t < 20000; ++t) {
+    std::vector<Holding> res(rng() % 7);
+    for (auto& h : res) h = {int64_t(1 + rng() % 20), 1 + rng() % 100, 256 * (1 + rng() % 8)};
+    const uint64_t cap = rng() % 3 ? 50 + rng() % 300 : 0;
+    const uint64_t budget = 256 * (8 + rng() % 40), fixed = 256 * (rng() % 4);
+    const uint64_t nn = 1 + rng() % 150, nb = 256 * (1 + rng() % 8);
+    int64_t want = m3lb::kNoHorizon;
+    for (int64_t step = 0; step <= 21; ++step) {
+      uint64_t nodes = 0, bytes = fixed;
+      for (auto& h : res)
+        if (h.steps_left > step) {
+          nodes += h.nodes;
+          bytes += h.bytes;
+        }
+      if ((cap == 0 || nodes == 0 || nodes + nn <= cap) && bytes + nb <= budget) {
+        want = step;
+        break;
+      }
+    }
+    CHECK(m3lb::fit_horizon(res, fixed, cap, budget, nn, nb) == want);
+  }
+  // may_pass: behind a head that does not fit, only the allowed ones are tried
+  {
+    m3lb::Admission q(64);
+    for (int i = 1; i <= 6; ++i) q.push(i);
+    std::vector<int64_t> asked;
+    auto got = q.run(
+        10, [&](int64_t id) { return id != 1; },
+        [&](int64_t head, int64_t id) {
+          CHECK(head == 1);
+          asked.push_back(id);
+          return id % 2 == 0;
+        });
+    CHECK((got == std::vector<int64_t>{2, 4, 6}));
+    CHECK((asked == std::vector<int64_t>{2, 3, 4, 5, 6}));
+    CHECK(q.size() == 3 && q.bypassed() == 3);
+  }
+  // lockstep simulation under a node cap
+  for (int trial = 0; trial < 200; ++trial) {
+    struct Req {
+      uint64_t nodes;
+      int64_t steps;
+    };
+    std::vector<Req> reqs(60);
+    for (auto& q : reqs) q = {1 + rng() % 120, int64_t(1 + rng() % 40)};
+    const uint64_t cap = 150 + rng() % 200;
+    m3lb::Admission adm(1000);
+    std::map<int64_t, int64_t> left;  // resident id -> steps left
+    std::map<int64_t, int64_t> due;   // waiting head -> latest admission step
+    size_t next = 0, finished = 0;
+    for (int64_t step = 0; finished < reqs.size() && step < 100000; ++step) {
+      while (next < reqs.size() && next <= size_t(step / 3)) adm.push(int64_t(next++));
+      uint64_t nodes = 0;
+      for (auto& kv : left) nodes += reqs[size_t(kv.first)].nodes;
+      auto holdings = [&]() {
+        std::vector<Holding> h;
+        for (auto& kv : left) h.push_back({kv.second, reqs[size_t(kv.first)].nodes, 0});
+        return h;
+      };
+      int64_t head = -1, horizon = 0;
+      auto got = adm.run(
+          1000,
+          [&](int64_t id) {
+            const uint64_t bn = reqs[size_t(id)].nodes;
+            if (nodes > 0 && nodes + bn > cap) return false;
+            nodes += bn;
+            left[id] = reqs[size_t(id)].steps;
+            return true;
+          },
+          [&](int64_t h, int64_t id) {
+            if (h != head) {
+              head = h;
+              horizon = m3lb::fit_horizon(holdings(), 0, cap, ~uint64_t(0) >> 1,
+                                          reqs[size_t(h)].nodes, 0);
+              if (!due.count(h)) due[h] = step + horizon;
+            }
+            return reqs[size_t(id)].steps <= horizon;
+          });
+      for (int64_t id : got) {
+        if (due.count(id)) CHECK(step <= due[id]);
+      }
+      // one step of every resident box
+      for (auto it = left.begin(); it != left.end();) {
+        if (--it->second == 0) {
+          it = left.erase(it);
+          ++finished;
+        } else {
+          ++it;
+        }
+      }
+    }
+    CHECK(finished == reqs.size());
+  }
+}
+
+// The chunk layout: descending steps (stable), prefix sums, and at every step
+// the stepping boxes are exactly a prefix whose tiles hdr records.
+static void pack_checks() {
+  std::mt19937_64 rng(5);
+  for (int t = 0; t < 5000; ++t) {
+    const int K = 1 + int(rng() % 12);
+    const size_t n = rng() % 20;
+    std::vector<unsigned> tiles(n);
+    std::vector<int> steps(n);
+    for (size_t k = 0; k < n; ++k) {
+      tiles[k] = unsigned(rng() % 50);
+      steps[k] = 1 + int(rng() % unsigned(K));
+    }
+    std::vector<unsigned> hdr(m3lb::pack_words(n, K), 0xDEADu);
+    std::vector<size_t> order;
+    m3lb::pack_chunk(tiles, steps, K, order, hdr.data());
+    CHECK(order.size() == n && hdr[0] == n);
+    std::set<size_t> seen(order.begin(), order.end());
+    CHECK(seen.size() == n);
+    unsigned total = 0;
+    for (size_t k = 0; k < n; ++k) {
+      CHECK(hdr[2 + k] == total);
+      total += tiles[order[k]];
+      if (k > 0) {
+        CHECK(steps[order[k - 1]] >= steps[order[k]]);
+        if (steps[order[k - 1]] == steps[order[k]]) CHECK(order[k - 1] < order[k]);
+      }
+    }
+    CHECK(hdr[1] == total && hdr[2 + n] == total);
+    for (int j = 0; j < K; ++j) {
+      unsigned want = 0;
+      for (size_t k = 0; k < n; ++k)
+        if (steps[k] > j) want += tiles[k];
+      const unsigned act = hdr[3 + n + size_t(j)];
+      CHECK(act == want);
+      // the tiles [0, act) are exactly those of the boxes stepping at j
+      for (size_t k = 0; k < n; ++k) {
+        const bool in = hdr[2 + k] < act || (tiles[order[k]] == 0 && hdr[2 + k] <= act &&
+                                             steps[order[k]] > j);
+        if (tiles[order[k]]) CHECK(in == (steps[order[k]] > j));
+      }
+    }
+  }
+}
+
 // A seeding failure during admission: the failing box and every box admitted
 // after it are handed back unseeded, in order, and the error comes back; the
 // boxes before it stay.  With an arena behind it, the memory of the handed-back
@@ -186,6 +342,8 @@ int main() {
   fastdiv_checks();
   arena_checks();
   admission_checks();
+  horizon_checks();
+  pack_checks();
   seed_failure_checks();
   if (failures) return 1;
   std::printf("ok\n");

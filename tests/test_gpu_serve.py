@@ -315,3 +315,34 @@ def test_resident_node_cap():
         rb = srv.submit(big)
         srv.drain()
         assert srv.result(rb)["state"] == abi.REQ_DONE
+
+
+def test_passing_never_delays_the_head():
+    """Under a node cap, a head H that does not fit beside resident box A
+    waits for A's 40 steps.  Behind it, S (8 steps) fits now and will have
+    finished by then, so it is admitted at once; L (200 steps) would hold its
+    nodes past that point, so it waits and is admitted after H.  S computes
+    what it computes alone."""
+    A, H = tg((30, 26, 22), 40), tg((40, 40, 40), 4)
+    S, L = tg((16, 16, 16), 8), tg((16, 16, 16), 200)
+    cap = 30 * 26 * 22 + 16 ** 3 + 100
+    scfg = abi.make_server_config(budget_bytes=256 * MiB, steps_per_chunk=4,
+                                  max_resident_nodes=cap)
+    ref = run_server(abi.make_server_config(budget_bytes=256 * MiB), [tg((16, 16, 16), 8)])[0]
+    with Server(scfg) as srv:
+        ids = []
+        for r in (A, H, S, L):
+            r.keep = 1
+            ids.append(srv.submit(r))
+        srv.progress(0)
+        st = srv.stats()
+        assert st["resident"] == 2 and st["queued"] == 2
+        srv.drain()
+        res = {rid: srv.result(rid) for rid in ids}
+        a, h, s_, l_ = (res[i] for i in ids)
+        assert all(r["state"] == abi.REQ_DONE for r in res.values())
+        assert s_["admitted"] <= h["admitted"] and s_["completed"] <= h["admitted"]
+        assert a["completed"] <= h["admitted"] <= l_["admitted"]
+        assert np.array_equal(srv.fetch_populations(ids[2]), ref[0])
+        assert s_["mass"] == ref[1]["mass"]
+

@@ -77,6 +77,7 @@ struct StepArgs {
   int remap;       // tile order, see tile_of()
   int cached;      // 1: plain accesses everywhere (a cache-resident working set)
   unsigned lds_cap;  // bytes of LDS a capped step launch reserves (0: none)
+  FastDiv div_ny;  // row -> (y, z) without an integer division
   double omega, omega_m, omega_b;
   double F[3];
   void* macro[4];  // rho, ux, uy, uz (compute type), or null
@@ -256,6 +257,15 @@ __host__ __device__ __forceinline__ unsigned tile_of(unsigned b, unsigned G, int
   const unsigned xcd = b & 7u, k = b >> 3, m = full >> 3;
   if (run < 0) return xcd * m + k;
   const unsigned r = unsigned(run);
+  if ((r & (r - 1u)) == 0u) {
+    // a power of two (every order the library picks): shifts, which stay in
+    // scalar instructions, where a division by a run-time value would take
+    // a vector reciprocal sequence per wave
+    const unsigned sh = unsigned(__builtin_ctz(r));
+    const unsigned whole = (m >> sh) << sh;
+    if (k < whole) return (((k >> sh) * 8u + xcd) << sh) + (k & (r - 1u));
+    return whole * 8u + xcd * (m - whole) + (k - whole);
+  }
   const unsigned whole = (m / r) * r;  // tiles per XCD dealt in whole rounds
   if (k < whole) return ((k / r) * 8u + xcd) * r + (k % r);
   return whole * 8u + xcd * (m - whole) + (k - whole);
@@ -294,9 +304,25 @@ __device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint6
   force_at<K>(p, ff, stride, n);
   const Offs o = offsets(d, x, y, z);
   R f[K::Q];
-  load_node<K, P, NTL>(A, stride, n, o, inv_scale, f);
+  uint64_t stride_st = stride;
+  if constexpr (K::C::scaled) {
+    // 16-bit storage: the step is bound by arithmetic, and the scalar
+    // registers of 27 direction bases and 54 scales do not fit at once (the
+    // compiler spills them through vector lanes, 60-70 extra vector
+    // instructions).  The scales are powers of two, so 1/s is 2^-k from
+    // the bits of s, and the store bases are recomputed after the collision
+    // (the empty asm makes the stride opaque from the point f[0] exists on),
+    // so neither stays live across it.
+    float inv[K::Q];
+    for (int q = 0; q < K::Q; ++q)
+      inv[q] = __builtin_bit_cast(float, 0x7F000000u - __builtin_bit_cast(uint32_t, scale[q]));
+    load_node<K, P, NTL>(A, stride, n, o, inv, f);
+  } else {
+    load_node<K, P, NTL>(A, stride, n, o, inv_scale, f);
+  }
   const Macro<R> m = collide_fused<K>(f, p);
-  store_node<K, P, NTL>(A, stride, n, o, scale, f);
+  if constexpr (K::C::scaled) asm volatile("" : "+s"(stride_st) : "v"(f[0]));
+  store_node<K, P, NTL>(A, stride_st, n, o, scale, f);
   if (macro && macro[0]) {
     static_cast<R*>(macro[0])[n] = density<K>(m);
     static_cast<R*>(macro[1])[n] = m.ux;
@@ -315,8 +341,9 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   const Dom& d = a.d;
   if (row >= a.rows || x >= d.nx) return;
   const Params<typename K::R> p = params_of<K>(a);
+  const int z = int(fd_div(a.div_ny, unsigned(row)));
   node_update<K, P, GEOM, NTL>(static_cast<typename K::T*>(a.f), a.stride, a.flags, d, x,
-                          row % d.ny, row / d.ny, p, a.scale, a.inv_scale, a.macro, a.ff);
+                          row - z * d.ny, z, p, a.scale, a.inv_scale, a.macro, a.ff);
 }
 
 // ---------------------------------------------------------------------------

@@ -139,9 +139,9 @@ struct Codec<M3LB_E2M13> {
   using R = float;
   using F = NarrowFloat16<2>;
   static constexpr bool scaled = true;
-  // decoding multiplies by the scale's inverse: the folded form (dec_scaled)
-  // holds two scalar constants per pair, and with them the step kernel's
-  // scheduler serialised its 27 loads to save vector registers
+  // decoding multiplies by the scale's inverse, which the compiler folds into
+  // the moment sums; the folded decoder (dec_scaled) measured more vector
+  // instructions in the step kernel
   __device__ __forceinline__ static R dec(T v, float inv) { return F::dec(v) * inv; }
   __device__ __forceinline__ static T enc(R v, float s) { return F::enc_scaled(v, s); }
 };
@@ -272,7 +272,11 @@ __device__ __forceinline__ Macro<typename K::R> macro_of(const RawMoments<typena
   Macro<R> m;
   m.dens = S.S0;
   const R rho = density<K>(m);
-  const R ir = R(1) / rho;
+  // 16-bit storage rounds every population to 14 bits or fewer: the hardware
+  // reciprocal (1 ulp) replaces the correctly rounded division there
+  R ir;
+  if constexpr (K::C::scaled) ir = __builtin_amdgcn_rcpf(rho);
+  else ir = R(1) / rho;
   m.ux = S.Sx * ir; m.uy = S.Sy * ir; m.uz = S.Sz * ir;
   if constexpr (K::FORCE) {
     const R h = R(0.5) * ir;
@@ -415,6 +419,20 @@ __device__ __forceinline__ void inv1d(R& a, R& b, R& c, R u, R u2m) {
   a = R(0.5) * (s2 - s1);
   b = m0 - s2;
   c = R(0.5) * (s2 + s1);
+}
+// The same inverse in seven operations (inv1d as written takes eight or
+// nine: the halvings of s2 - s1 and s2 + s1 become one product and two fused
+// multiply-adds).  Equal to inv1d up to rounding; the 16-bit storage forms,
+// whose step is bound by arithmetic, use it.  u2 = 2 u.
+template <class R>
+__device__ __forceinline__ void inv1d_fast(R& a, R& b, R& c, R u, R u2, R u2m) {
+  const R m0 = a;
+  const R s1 = __builtin_fmaf(u, m0, b);
+  const R s2 = __builtin_fmaf(u2, s1, __builtin_fmaf(-u2m, m0, c));
+  const R hs1 = R(0.5) * s1;
+  a = __builtin_fmaf(R(0.5), s2, -hs1);
+  b = m0 - s2;
+  c = __builtin_fmaf(R(0.5), s2, hs1);
 }
 
 // MomentCollision::collide, MomentCollision.hpp:234-314; CENTRAL selects u_b = u.
@@ -585,7 +603,9 @@ __device__ __forceinline__ Macro<typename K::R> collide_moment_fused(typename K:
   R ub[3], du[3], u2m[3];
   for (int a = 0; a < 3; ++a) {
     ub[a] = CENTRAL ? u[a] : R(0);
-    du[a] = u[a] - ub[a];
+    // u - u_b, written as the constant 0 for central moments: the same value
+    // for every finite u, and the terms it multiplies then fold away
+    du[a] = CENTRAL ? R(0) : u[a];
     u2m[a] = ub[a] * ub[a] - cs2;
   }
   R Qf[3][3], Aw[3][3];
@@ -663,7 +683,24 @@ __device__ __forceinline__ Macro<typename K::R> collide_moment_fused(typename K:
     constexpr int ord = D == 2 ? (n / 3 + n % 3) : (n / 9 + (n / 3) % 3 + n % 3);
     if constexpr (ord >= 3) k[n] = eqm(N);
   });
-  if constexpr (D == 3) {
+  if constexpr (D == 3 && K::C::scaled && std::is_same_v<R, float>) {
+    const R u2[3] = {R(2) * ub[0], R(2) * ub[1], R(2) * ub[2]};
+    sfor<9>([&](auto QR) {
+      constexpr int q = QR / 3, r = QR % 3;
+      inv1d_fast(k[mslot<3>(0, q, r)], k[mslot<3>(1, q, r)], k[mslot<3>(2, q, r)], ub[0], u2[0],
+                 u2m[0]);
+    });
+    sfor<9>([&](auto AR) {
+      constexpr int a = AR / 3, r = AR % 3;
+      inv1d_fast(k[mslot<3>(a, 0, r)], k[mslot<3>(a, 1, r)], k[mslot<3>(a, 2, r)], ub[1], u2[1],
+                 u2m[1]);
+    });
+    sfor<9>([&](auto AB) {
+      constexpr int a = AB / 3, b = AB % 3;
+      inv1d_fast(k[mslot<3>(a, b, 0)], k[mslot<3>(a, b, 1)], k[mslot<3>(a, b, 2)], ub[2], u2[2],
+                 u2m[2]);
+    });
+  } else if constexpr (D == 3) {
     sfor<9>([&](auto QR) {
       constexpr int q = QR / 3, r = QR % 3;
       inv1d(k[mslot<3>(0, q, r)], k[mslot<3>(1, q, r)], k[mslot<3>(2, q, r)], ub[0], u2m[0]);

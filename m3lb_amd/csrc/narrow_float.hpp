@@ -55,9 +55,12 @@ struct NarrowFloat16 {
   M3LB_HD static float dec(uint16_t c) {
     // the sign bit lands below the mantissa's last kept bit and is cleared
     // after the add (a shift-add is one instruction, a masked shift is not)
-    const float g = __builtin_bit_cast(float, ((uint32_t(c) << (SHIFT - 1)) + BASE) & ~(1u << (SHIFT - 1)));
-    const float mag = __builtin_fminf(__builtin_fmaf(g, 2.0f, -P_MIN), g);
-    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, mag) | (uint32_t(c) << 31));
+    const uint32_t gb = ((uint32_t(c) << (SHIFT - 1)) + BASE) & ~(1u << (SHIFT - 1));
+    // both operands are non-negative (g >= 2^(EMIN-1)), so the smaller float
+    // has the smaller bit pattern: an integer min needs no quieting of NaNs
+    const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_fmaf(__builtin_bit_cast(float, gb), 2.0f, -P_MIN));
+    const uint32_t mag = lo < gb ? lo : gb;
+    return __builtin_bit_cast(float, mag | (uint32_t(c) << 31));
   }
 
   // enc(x * s) and dec(c) * inv for a power-of-two scale s = 1 / inv, with
@@ -84,8 +87,9 @@ struct NarrowFloat16 {
     const uint32_t gb = ((uint32_t(c) << (SHIFT - 1)) + (BASE + kb)) & ~(1u << (SHIFT - 1));
     const float g = __builtin_bit_cast(float, gb);
     const float p = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, P_MIN) + kb);  // 2^EMIN inv
-    const float mag = __builtin_fminf(__builtin_fmaf(g, 2.0f, -p), g);
-    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, mag) | (uint32_t(c) << 31));
+    const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_fmaf(g, 2.0f, -p));
+    const uint32_t mag = lo < gb ? lo : gb;  // non-negative: as in dec()
+    return __builtin_bit_cast(float, mag | (uint32_t(c) << 31));
   }
 };
 

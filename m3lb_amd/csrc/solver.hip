@@ -199,6 +199,7 @@ m3lb::StepArgs step_args(const m3lb_solver* s) {
   a.bx_log2 = nx > 128 ? 8 : (nx > 64 ? 7 : 6);
   a.chunks = (nx + m3lb::kBlock - 1) / m3lb::kBlock;
   a.rows = s->d.ny * s->d.nz;
+  a.div_ny = m3lb::make_fastdiv(uint32_t(s->d.ny));
   a.remap = s->remap;
   a.lds_cap = s->lds_cap;
   const uint64_t bytes = s->stride * uint64_t(s->Q) * uint64_t(s->ks->elem_bytes);

@@ -276,8 +276,11 @@ __host__ __device__ __forceinline__ unsigned tile_of(unsigned b, unsigned G, int
 __device__ __forceinline__ void tile_node(unsigned tile, int chunks, int bx_log2, int& row,
                                           int& x) {
   if (chunks > 1) {
-    row = int(tile / unsigned(chunks));
-    x = int(tile - unsigned(row) * unsigned(chunks)) * kBlock + int(threadIdx.x);
+    const unsigned c = unsigned(chunks);
+    // a power of two (every box edge up to 1024 that is a multiple of 256,
+    // e.g. 512: two chunks) needs a shift, not a run-time division
+    row = (c & (c - 1u)) == 0u ? int(tile >> unsigned(__builtin_ctz(c))) : int(tile / c);
+    x = int(tile - unsigned(row) * c) * kBlock + int(threadIdx.x);
   } else {
     row = int(tile) * (kBlock >> bx_log2) + int(threadIdx.x >> bx_log2);
     x = int(threadIdx.x & ((1u << bx_log2) - 1u));
@@ -336,11 +339,27 @@ __device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint6
 // caches).
 template <class K, int P, bool GEOM, bool NTL>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
-  int row, x;
-  tile_node(tile_of(blockIdx.x, gridDim.x, a.remap), a.chunks, a.bx_log2, row, x);
   const Dom& d = a.d;
-  if (row >= a.rows || x >= d.nx) return;
+  const unsigned tile = tile_of(blockIdx.x, gridDim.x, a.remap);
   const Params<typename K::R> p = params_of<K>(a);
+  if (a.chunks > 1) {
+    // rows of more than one 256-node chunk: the whole block is on one row,
+    // so row, y, z and the row part of every address stay scalar (a copy of
+    // the node body that saves about 30 vector instructions per node; the
+    // rows-per-block path below has them per lane)
+    const unsigned c = unsigned(a.chunks);
+    const unsigned sh = unsigned(__builtin_ctz(c));
+    const int row = (c & (c - 1u)) == 0u ? int(tile >> sh) : int(tile / c);
+    const int x = int(tile - unsigned(row) * c) * kBlock + int(threadIdx.x);
+    if (row >= a.rows || x >= d.nx) return;
+    const int z = int(fd_div(a.div_ny, unsigned(row)));
+    node_update<K, P, GEOM, NTL>(static_cast<typename K::T*>(a.f), a.stride, a.flags, d, x,
+                            row - z * d.ny, z, p, a.scale, a.inv_scale, a.macro, a.ff);
+    return;
+  }
+  int row, x;
+  tile_node(tile, a.chunks, a.bx_log2, row, x);
+  if (row >= a.rows || x >= d.nx) return;
   const int z = int(fd_div(a.div_ny, unsigned(row)));
   node_update<K, P, GEOM, NTL>(static_cast<typename K::T*>(a.f), a.stride, a.flags, d, x,
                           row - z * d.ny, z, p, a.scale, a.inv_scale, a.macro, a.ff);

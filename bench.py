@@ -25,6 +25,8 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, published
 FULL_SERVING_REQUESTS = 4000  # the serving workload's stated request count
+FUSION_MODES = {"auto": lambda abi: abi.FUSION_AUTO, "off": lambda abi: abi.FUSION_OFF,
+                "on": lambda abi: abi.FUSION_ON}
 
 
 def log(*a):
@@ -44,12 +46,14 @@ def bytes_per_node(abi, dtype, Q=27):
                     abi.E2M13: 2}[dtype]
 
 
-def gpu_leg(D, steps, warmup, dtype, op, seed=3, field=None):
+def gpu_leg(D, steps, warmup, dtype, op, seed=3, field=None, fusion=None):
     from m3lb_amd import abi, cases
     from m3lb_amd.solver import FluidSolver
     cfg, u0 = taylor_green_config(abi, D, dtype, op)
     t0 = time.time()
     s = FluidSolver(cfg)
+    if fusion is not None:
+        s.set_step_fusion(fusion)
     if field is None:
         field = cases.tgv3d_benchmark(cfg, u0, perturbation=1e-4, seed=seed)
     s.initialize_field(*field)
@@ -375,6 +379,9 @@ def main():
                     help="timed CPU steps after one warm-up (a bounded sample)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fusion", default="auto", choices=sorted(FUSION_MODES),
+                    help="two steps per pass: the library's choice (auto), never (off: one "
+                         "launch per step) or whenever the box allows (on)")
     ap.add_argument("--extra", action="store_true",
                     help="also time the 16-bit and FP64 storage forms and BGK (stderr)")
     ap.add_argument("--serving-requests", type=int, default=200,
@@ -440,7 +447,8 @@ def main():
     from m3lb_amd import cases
     cfg0, u00 = taylor_green_config(abi, args.size, abi.F32, abi.OP_CM)
     field = cases.tgv3d_benchmark(cfg0, u00, perturbation=1e-4, seed=3)
-    r = gpu_leg(args.size, args.steps, args.warmup, dt, abi.OP_CM, field=field)
+    r = gpu_leg(args.size, args.steps, args.warmup, dt, abi.OP_CM, field=field,
+                fusion=FUSION_MODES[args.fusion](abi))
     log(f"[bench] gpu {args.size}^3 CM {args.dtype}: {r['mlups']:.0f} MLUPS, "
         f"{r['ms_per_step']:.3f} ms/step, {r['achieved_gbs']:.0f} GB/s, "
         f"mass drift {r['mass_drift']:.2e}, finite {r['finite']}")
@@ -494,7 +502,7 @@ def main():
     traffic, traffic_src = None, None
     tj = os.path.join(ROOT, "profiles", "k_pair_cm_f32_512cubed_traffic.json")
     build = solver.kernel_build_id()
-    if args.size == 512 and args.dtype == "f32" and os.path.exists(tj):
+    if args.size == 512 and args.dtype == "f32" and args.fusion != "off" and os.path.exists(tj):
         with open(tj) as fh:
             tr = json.load(fh)
         if tr.get("build_id") == build:
@@ -541,7 +549,8 @@ def main():
         "config": {"workload": f"D3Q27 central-moment esoteric-pull step, {args.dtype} shifted "
                                f"populations, {args.size}^3 periodic Taylor-Green u0=0.04 "
                                f"Re=1600 + seeded 1e-4 perturbation",
-                   "nodes": args.size ** 3, "bytes_per_node_step": bpn},
+                   "nodes": args.size ** 3, "bytes_per_node_step": bpn,
+                   "step_fusion": args.fusion},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic},

@@ -82,6 +82,18 @@ def test_pair_is_bit_identical_16bit(dtype):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("dtype", [abi.F16, abi.E2M13])
+def test_pair_is_bit_identical_16bit_wide_rows(dtype):
+    """Rows of two 256-node chunks: one launch per step takes its
+    scalar-row path there, the pass its own node body; the bits agree."""
+    shape = (300, 16, 10)
+    cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dtype, shape, omega=1.6)
+    a, _, _ = run(cfg, 8, abi.FUSION_OFF)
+    b, nb, _ = run(cfg, 8, abi.FUSION_ON, lag=2)
+    assert nb == 4
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("shape,periodic", [((70, 30, 12), (1, 0, 0)), ((300, 12, 9), (1, 1, 1)),
                                             ((33, 19, 21), (0, 1, 0))])
 @pytest.mark.parametrize("stripes,lag,bpc,group", [(0, 0, 0, 0), (3, 1, 2, 1), (5, 4, 8, 3)])

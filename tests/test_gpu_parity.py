@@ -194,6 +194,61 @@ def test_step_matches_oracle(lat, op, dtype, storage, force):
             assert abs(s.total_momentum(a) - o.total_momentum(a)) <= (1e-13 if dtype == abi.F64 else 1e-6) * max(1.0, abs(mo))
 
 
+@pytest.mark.parametrize("shape", [(300, 10, 8), (530, 6, 5)], ids=["chunks2", "chunks3"])
+@pytest.mark.parametrize("dtype", [abi.F64, abi.F32])
+@pytest.mark.parametrize("op", [abi.OP_BGK, abi.OP_CM])
+def test_wide_rows_match_oracle(shape, dtype, op):
+    """Rows of more than one 256-node chunk run the step's scalar-row path
+    (two chunks: a shift; three: a division): walls in y, an obstacle, Guo
+    force, 21 steps against the oracle at the tolerances above."""
+    nx, ny, nz = shape
+    cfg = abi.make_config(abi.D3Q27, op, abi.SHIFTED, dtype, shape, periodic=(True, False, True),
+                          omega=1.3, omega_bulk=1.1 if op == abi.OP_CM else -1.0,
+                          force=(2e-5, -1e-5, 3e-6))
+    fl = np.zeros((nz, ny, nx), dtype=np.uint8)
+    fl[:, 0, :] = abi.SOLID
+    fl[:, ny - 1, :] = abi.SOLID
+    fl[:, ny // 2, nx // 3:nx // 3 + 3] = abi.SOLID
+    o = OracleSolver(cfg)
+    o.set_geometry(fl)
+    x, y, z, d = cases.padded_coords(cfg)
+    field = [1.0 + 0.02 * np.sin(0.05 * x + 0.3 * z), 0.04 * np.cos(0.5 * y + 0.01 * x),
+             0.02 * np.sin(0.04 * x), 0.01 * np.cos(0.03 * x + 0.2 * y)]
+    o.initialize_field(*field)
+    with FluidSolver(cfg) as s:
+        s.set_geometry(fl)
+        s.initialize_field(*field)
+        o.step(21)
+        s.step(21)
+        _, uxo, uyo, uzo = o.compute_macroscopic()
+        _, uxs, uys, uzs = s.compute_macroscopic()
+        num = np.sum((uxs - uxo) ** 2 + (uys - uyo) ** 2 + (uzs - uzo) ** 2)
+        den = np.sum(uxo ** 2 + uyo ** 2 + uzo ** 2)
+        assert np.sqrt(num / den) <= (1e-12 if dtype == abi.F64 else 1e-5)
+        gp, go = s.gather_populations(), o.gather_populations()
+        assert np.abs(gp - go).max() <= np.abs(go).max() * (1e-13 if dtype == abi.F64 else 2e-5)
+    o.close()
+
+
+@pytest.mark.parametrize("dtype", [abi.F16, abi.E2M13])
+def test_wide_rows_16bit_match_fp64_storage(dtype):
+    """The 16-bit step on rows of two chunks (the path the 512^3 benchmark
+    takes) against FP64 storage of the same Taylor-Green box, 100 steps.
+    (y and z extents of 32 and 24 keep the flow from decaying to velocities
+    at the 16-bit resolution itself.)"""
+    shape = (512, 32, 24)
+    out = {}
+    for dt in (abi.F64, dtype):
+        cfg = abi.make_config(abi.D3Q27, abi.OP_CM, abi.SHIFTED, dt, shape, omega=1.6)
+        with FluidSolver(cfg) as s:
+            s.initialize_taylor_green(0.04)
+            s.step(100)
+            out[dt] = s.compute_macroscopic()[1:]
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(out[dtype], out[abi.F64]))
+    den = sum(float(np.sum(b ** 2)) for b in out[abi.F64])
+    assert np.sqrt(num / den) <= 1e-3
+
+
 def test_store_macroscopic_matches_compute():
     """step(store_macroscopic = true) writes the macroscopic state of the
     populations the step read (FluidSolver.hpp:1033-1036), which is what

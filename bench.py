@@ -25,6 +25,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, published
 FULL_SERVING_REQUESTS = 4000  # the serving workload's stated request count
+DTYPES = {"f64": lambda abi: abi.F64, "f32": lambda abi: abi.F32, "f16": lambda abi: abi.F16,
+          "bf16": lambda abi: abi.BF16, "i16": lambda abi: abi.I16,
+          "e2m13": lambda abi: abi.E2M13}
 FUSION_MODES = {"auto": lambda abi: abi.FUSION_AUTO, "off": lambda abi: abi.FUSION_OFF,
                 "on": lambda abi: abi.FUSION_ON}
 
@@ -120,14 +123,15 @@ def _pct(v, q):
 
 
 def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk, max_resident_nodes=0,
-                    use_graph=True):
+                    use_graph=True, dtype=None):
     """The requests through one batched server: arrivals replayed on the
     server clock, continuous admission, completion times from the server."""
     from m3lb_amd import abi
     from m3lb_amd.serve import Server
     scfg = abi.make_server_config(budget_bytes=budget_bytes, max_boxes=4096,
                                   steps_per_chunk=steps_per_chunk,
-                                  max_resident_nodes=max_resident_nodes, use_graph=use_graph)
+                                  max_resident_nodes=max_resident_nodes, use_graph=use_graph,
+                                  dtype=abi.F32 if dtype is None else dtype)
     n = len(work)
     with Server(scfg) as srv:
         tune = srv.autotune(steps=tune_steps) if tune_steps > 0 else None
@@ -165,13 +169,13 @@ def serving_batched(work, budget_bytes, tune_steps, steps_per_chunk, max_residen
             "tune": tune, "lattice_updates": lu}
 
 
-def serving_per_case(work):
+def serving_per_case(work, dtype=None):
     """The same requests one case at a time, in arrival order, each on its own
     single-box solver: create, geometry, initialise, one launch per step,
     read the mass (synchronises), destroy."""
     from m3lb_amd import abi, cases
     from m3lb_amd.solver import FluidSolver
-    scfg = abi.make_server_config()
+    scfg = abi.make_server_config(dtype=abi.F32 if dtype is None else dtype)
     t0 = time.perf_counter() + 0.01
     free, ttc, step_s = t0, [], 0.0
     next_log = t0 + 20.0
@@ -325,16 +329,17 @@ def configs_leg():
 
 
 def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0, use_graph=True,
-                edges=(32, 192)):
-    from m3lb_amd import cases
+                edges=(32, 192), dtype_name="f32"):
+    from m3lb_amd import abi, cases
+    dtype = DTYPES[dtype_name](abi)
     work = cases.serving_workload(n, seed=seed, rate=rate, edges=edges)
     b = serving_batched(work, int(budget_gb * 1e9), tune_steps, steps_per_chunk, cap,
-                        use_graph)
+                        use_graph, dtype)
     log(f"[bench] serving batched: {b['mlups']:.0f} MLUPS, p50 {b['p50_s']:.2f} s, "
         f"p99 {b['p99_s']:.2f} s, makespan {b['makespan_s']:.1f} s, peak resident "
         f"{b['peak_resident']}")
     work = cases.serving_workload(n, seed=seed, rate=rate, edges=edges)
-    c = serving_per_case(work)
+    c = serving_per_case(work, dtype)
     log(f"[bench] serving one launch per case: {c['mlups']:.0f} MLUPS, p50 {c['p50_s']:.2f} s, "
         f"p99 {c['p99_s']:.2f} s, makespan {c['makespan_s']:.1f} s")
     rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
@@ -342,7 +347,7 @@ def serving_leg(n, rate, seed, budget_gb, tune_steps, steps_per_chunk, cap=0, us
               f"{FULL_SERVING_REQUESTS}-request workload is not measured by this run"
               if n < FULL_SERVING_REQUESTS and edges == (32, 192) else f"{n} requests")
     return {"requests": n, "arrival_rate_per_s": rate, "seed": seed,
-            "workload": "D3Q27 CM FP32 shifted; 50% periodic Taylor-Green u0=0.04, 50% "
+            "workload": f"D3Q27 CM {dtype_name.upper()} shifted; 50% periodic Taylor-Green u0=0.04, 50% "
                         "bounce-back channels g_x~U[1e-7,1e-5]; edges log-uniform "
                         f"[{edges[0]},{edges[1]}) "
                         f"per axis; tau~U[0.51,0.8]; steps~U{{200..3000}}; Poisson arrivals; "
@@ -398,6 +403,8 @@ def main():
                     help="requests of the small-box serving workload (edges [16, 48)); 0 skips it")
     ap.add_argument("--serving-small-rate", type=float, default=400.0, help="arrivals per second")
     ap.add_argument("--serving-only", action="store_true")
+    ap.add_argument("--serving-dtype", default="f32", choices=sorted(DTYPES),
+                    help="storage of the served boxes (--serving-only; 16-bit forms shifted)")
     ap.add_argument("--no-e2m13", action="store_true",
                     help="skip the E2M13-storage run of the same workload")
     ap.add_argument("--e2m13-steps", type=int, default=2000,
@@ -438,7 +445,7 @@ def main():
     if args.serving_only:
         sv = serving_leg(args.serving_requests, args.serving_rate, 4, args.serving_budget_gb,
                          args.serving_tune_steps, args.serving_chunk, args.serving_cap,
-                         not args.serving_no_graph)
+                         not args.serving_no_graph, dtype_name=args.serving_dtype)
         small = serving_small(args)
         print(json.dumps({"serving": sv, "serving_small_boxes": small}), flush=True)
         return

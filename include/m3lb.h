@@ -400,7 +400,9 @@ int m3lb_initialize_taylor_green(m3lb_solver* s, double u0);
  * passing never delays the head; at most max_bypass pass one head.  A box computes bit-identical results alone, in any mix, and
  * in the single-box solver with the same configuration and forcing policy.
  * Every box of a server shares one kernel configuration (lattice, operator,
- * storage, dtype, forcing policy); F32 and F64 storage only.
+ * storage, dtype, forcing policy).  16-bit storage (shifted only) chooses
+ * each box's scales at admission as m3lb_initialize does (seed, measure,
+ * seed again), which synchronises the server's stream once per admitted box.
  * One host thread per server; calls are ordered on the server's own stream.
  * ------------------------------------------------------------------------ */
 enum { M3LB_SHAPE_ROWS = 0, M3LB_SHAPE_LINEAR = 1, M3LB_SHAPE_AUTO = 2 };

@@ -29,7 +29,7 @@ struct KernelSet {
   void (*snapshot)(const SnapArgs&, int parity, hipStream_t);
   // specular ghosts and on-node specular nodes
   void (*specular)(const SpecArgs&, int parity, hipStream_t);
-  // batched server step; null for the identity op and 16-bit storage
+  // batched server step; null for the identity op and a per-node force
   void (*batched)(const BatchArgs&, unsigned grid, hipStream_t);
   // two steps in one pass (pair.hpp): launch 1 on a persistent grid, then the
   // deferred rows; null where not instantiated (D2Q9, per-node force)
@@ -43,7 +43,7 @@ inline unsigned blocks_for(long long n) { return unsigned((n + kBlock - 1) / kBl
 
 template <class K>
 struct Launch {
-  static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::C::scaled && !K::FIELD;
+  static constexpr bool kBatched = K::OP != M3LB_OP_NULL && !K::FIELD;
   // Occupancy cap of the streaming step: the kernel uses no LDS, but a launch
   // that reserves a.lds_cap bytes of it per block (a quarter of the CU's LDS
   // plus one byte, set up by the solver from the device's attributes) fits at

@@ -24,6 +24,7 @@
 
 #include "launch.hpp"
 #include "serve_plan.hpp"
+#include "scales.hpp"
 
 using m3lb::BoxDesc;
 using m3lb::KernelSet;
@@ -86,6 +87,7 @@ struct Box {
   int slot = -1;
   double t_arrival = 0, t_admit = -1, t_done = -1;
   double summary[4] = {0, 0, 0, 0};
+  float scale[27] = {}, inv_scale[27] = {};  // 16-bit storage: chosen at seeding (scales.hpp)
   BoxDesc desc;
   unsigned tiles = 0;
 };
@@ -145,8 +147,9 @@ int validate(const m3lb_server_config* c) {
     return err(M3LB_EINVAL, "the server runs the collision operators BGK .. CM");
   if (c->storage != M3LB_RAW && c->storage != M3LB_SHIFTED)
     return err(M3LB_EINVAL, "storage must be M3LB_RAW or M3LB_SHIFTED");
-  if (c->dtype != M3LB_F64 && c->dtype != M3LB_F32)
-    return err(M3LB_EINVAL, "the server stores F32 or F64 populations");
+  if (c->dtype < M3LB_F64 || c->dtype > M3LB_E2M13) return err(M3LB_EINVAL, "unknown dtype");
+  if (m3lb::is_16bit(c->dtype) && c->storage != M3LB_SHIFTED)
+    return err(M3LB_EINVAL, "16-bit storage holds shifted populations (g = f - w) only");
   if (c->forcing != M3LB_FORCE_NONE && c->forcing != M3LB_FORCE_GUO)
     return err(M3LB_EINVAL, "forcing must be M3LB_FORCE_NONE or M3LB_FORCE_GUO");
   if (c->max_boxes < 1 || c->max_boxes > 65536) return err(M3LB_EINVAL, "max_boxes must be in 1..65536");
@@ -184,7 +187,9 @@ int request_domain(const m3lb_server_config& c, const m3lb_request* r, m3lb::Dom
   return M3LB_OK;
 }
 
-int elem_bytes(const m3lb_server_config& c) { return c.dtype == M3LB_F64 ? 8 : 4; }
+int elem_bytes(const m3lb_server_config& c) {
+  return c.dtype == M3LB_F64 ? 8 : (m3lb::is_16bit(c.dtype) ? 2 : 4);
+}
 
 void footprint_of(const m3lb_server_config& c, const m3lb::Dom& d, uint64_t& stride,
                   uint64_t& fbytes, uint64_t& total) {
@@ -233,7 +238,11 @@ void describe(const m3lb_server* s, Box& b) {
   x.omega_m = b.req.omega_minus;
   x.omega_b = b.req.omega_bulk;
   for (int k = 0; k < 3; ++k) x.F[k] = s->cfg.forcing == M3LB_FORCE_GUO ? b.req.force[k] : 0.0;
-  for (int q = 0; q < 27; ++q) x.scale[q] = x.inv_scale[q] = 1.0f;
+  const bool scaled = m3lb::is_16bit(s->cfg.dtype);  // the scales are set at seeding
+  for (int q = 0; q < 27; ++q) {
+    x.scale[q] = scaled ? b.scale[q] : 1.0f;
+    x.inv_scale[q] = scaled ? b.inv_scale[q] : 1.0f;
+  }
   if (x.shape == m3lb::kShapeLinear) {
     b.tiles = (x.nodes + m3lb::kBlock - 1) / m3lb::kBlock;
   } else {
@@ -255,7 +264,8 @@ m3lb::ReadArgs read_args(const m3lb_server* s, const Box& b, double* out) {
   a.omega_m = b.req.omega_minus;
   a.omega_b = b.req.omega_bulk;
   for (int k = 0; k < 3; ++k) a.F[k] = b.desc.F[k];
-  for (int q = 0; q < 27; ++q) a.inv_scale[q] = 1.0f;
+  const bool scaled = m3lb::is_16bit(s->cfg.dtype);
+  for (int q = 0; q < 27; ++q) a.inv_scale[q] = scaled ? b.inv_scale[q] : 1.0f;
   return a;
 }
 
@@ -294,7 +304,34 @@ int seed_box(m3lb_server* s, Box& b) {
     a.rho0 = b.req.rho0;
     for (int k = 0; k < 3; ++k) a.u0[k] = b.req.u[k];
   }
-  for (int q = 0; q < 27; ++q) a.scale[q] = 1.0f;
+  // 16-bit storage: as the single-box solver does, seed at provisional
+  // scales, measure the largest |g_q| of the field, and seed again at the
+  // scales that calls for (scales.hpp), so a box gets the solver's bits
+  m3lb::provisional_scales(s->cfg.dtype, s->Q, b.scale, b.inv_scale);
+  for (int q = 0; q < 27; ++q) a.scale[q] = b.scale[q];
+  s->ks->seed(a, s->stream);
+  int rc = launch_ok();
+  if (rc || !m3lb::is_16bit(s->cfg.dtype)) return rc;
+  const unsigned g = std::max(1u, std::min(kSumGrid, m3lb::blocks_for(b.d.np)));
+  m3lb::ReadArgs r = read_args(s, b, s->partial_d);
+  s->ks->absmax(r, g, s->stream);
+  rc = launch_ok();
+  if (rc) return rc;
+  std::vector<double> h(size_t(g) * size_t(s->Q));
+  SRV_HIP(hipMemcpyAsync(h.data(), s->partial_d, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         s->stream));
+  SRV_HIP(hipStreamSynchronize(s->stream));
+  std::vector<double> mx(size_t(s->Q), 0.0), want(size_t(s->Q), 1.0);
+  for (unsigned k = 0; k < g; ++k)
+    for (int q = 0; q < s->Q; ++q) mx[size_t(q)] = std::max(mx[size_t(q)], h[size_t(k) * s->Q + q]);
+  m3lb::pair_scales(s->cfg.dtype, s->Q, mx.data(), want.data());
+  for (int q = 0; q < s->Q; ++q) {
+    b.scale[q] = float(want[size_t(q)]);
+    b.inv_scale[q] = float(1.0 / want[size_t(q)]);
+    a.scale[q] = b.scale[q];
+    b.desc.scale[q] = b.scale[q];
+    b.desc.inv_scale[q] = b.inv_scale[q];
+  }
   s->ks->seed(a, s->stream);
   return launch_ok();
 }
@@ -662,7 +699,9 @@ int m3lb_srv_create(const m3lb_server_config* cfg, m3lb_server** out) {
         hipMalloc(&s->hdr_d[k], m3lb::pack_words(nb, kMaxChunk) * sizeof(unsigned)) != hipSuccess)
       return bail(err(M3LB_ENOMEM, "device descriptor buffers"));
   }
-  if (hipMalloc(&s->partial_d, kSumGrid * 4 * sizeof(double)) != hipSuccess ||
+  // completion sums (4 per block) and, for 16-bit storage, the largest
+  // |g_q| per block at seeding (Q per block)
+  if (hipMalloc(&s->partial_d, kSumGrid * 27 * sizeof(double)) != hipSuccess ||
       hipMalloc(&s->summary_d, nb * 4 * sizeof(double)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&s->summary_h), nb * 4 * sizeof(double), hipHostMallocDefault) != hipSuccess)
     return bail(err(M3LB_ENOMEM, "summary buffers"));

@@ -346,3 +346,22 @@ def test_passing_never_delays_the_head():
         assert np.array_equal(srv.fetch_populations(ids[2]), ref[0])
         assert s_["mass"] == ref[1]["mass"]
 
+
+
+@pytest.mark.parametrize("dtype", [abi.E2M13, abi.F16])
+def test_16bit_boxes_match_single_solver(dtype):
+    """16-bit boxes (shifted storage): each box gets the scales the
+    single-box solver chooses for the same request (seed, measure, seed
+    again) and then the same bits, alone or in a mix of Taylor-Green boxes
+    and forced channels; completion sums likewise."""
+    scfg = abi.make_server_config(dtype=dtype, budget_bytes=256 * MiB, steps_per_chunk=4)
+    reqs = [tg((32, 24, 20), 13), channel((30, 26, 18), 9), tg((300, 9, 7), 6, omega=1.9)]
+    got = run_server(scfg, [r for r in reqs], between=1)
+    for k, r in enumerate([tg((32, 24, 20), 13), channel((30, 26, 18), 9),
+                           tg((300, 9, 7), 6, omega=1.9)]):
+        pops, mass, mom = solo(scfg, r)
+        assert np.array_equal(got[k][0], pops), k
+        # the sums reduce in a different order from the solver's
+        assert abs(got[k][1]["mass"] - mass) <= 1e-12 * abs(mass), k
+        for a in range(3):
+            assert abs(got[k][1]["momentum"][a] - mom[a]) <= 1e-12 * abs(mass), k

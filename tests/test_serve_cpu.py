@@ -60,7 +60,8 @@ def _cfg(**kw):
 
 @pytest.mark.parametrize("shape,periodic,dtype", [((32, 32, 32), (1, 1, 1), abi.F32),
                                                   ((33, 50, 191), (1, 0, 1), abi.F32),
-                                                  ((64, 64, 1), (1, 1, 1), abi.F64)])
+                                                  ((64, 64, 1), (1, 1, 1), abi.F64),
+                                                  ((40, 24, 16), (1, 0, 1), abi.E2M13)])
 def test_footprint(shape, periodic, dtype):
     lat = abi.D2Q9 if shape[2] == 1 else abi.D3Q27
     scfg = _cfg(lattice=lat, dtype=dtype)
@@ -68,13 +69,14 @@ def test_footprint(shape, periodic, dtype):
     d = abi.domain_of(abi.request_config(scfg, req))
     np_ = d[9]
     stride = (np_ + 63) // 64 * 64
-    elem = 8 if dtype == abi.F64 else 4
+    elem = {abi.F64: 8, abi.F32: 4}.get(dtype, 2)
     rnd = lambda b: (b + 255) // 256 * 256
     assert serve.footprint(scfg, req) == rnd(stride * lat * elem) + rnd(np_)
 
 
 @pytest.mark.parametrize("bad", [
-    dict(op=abi.OP_NULL), dict(dtype=abi.F16), dict(dtype=abi.BF16), dict(max_boxes=0),
+    dict(op=abi.OP_NULL), dict(dtype=abi.F16, storage=abi.RAW), dict(dtype=abi.E2M13, storage=abi.RAW),
+    dict(dtype=9), dict(max_boxes=0),
     dict(steps_per_chunk=0), dict(steps_per_chunk=2000), dict(blocks_per_cu=0),
     dict(tile_shape=7), dict(max_bypass=-1), dict(budget_bytes=0), dict(forcing=5),
     dict(tile_run=-3),

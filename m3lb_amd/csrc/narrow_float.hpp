@@ -60,7 +60,15 @@ struct NarrowFloat16 {
     // has the smaller bit pattern: an integer min needs no quieting of NaNs
     const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_fmaf(__builtin_bit_cast(float, gb), 2.0f, -P_MIN));
     const uint32_t mag = lo < gb ? lo : gb;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // one shift-or: written as two operations, the compiler splits most of
+    // them into a shift and an or (one more vector instruction per decode)
+    uint32_t r;
+    asm("v_lshl_or_b32 %0, %1, 31, %2" : "=v"(r) : "v"(uint32_t(c)), "v"(mag));
+    return __builtin_bit_cast(float, r);
+#else
     return __builtin_bit_cast(float, mag | (uint32_t(c) << 31));
+#endif
   }
 
   // enc(x * s) and dec(c) * inv for a power-of-two scale s = 1 / inv, with

@@ -84,9 +84,32 @@ __device__ __forceinline__ void node_update_hinted(typename K::T* __restrict__ A
   force_at<K>(p, a.ff, a.stride, n);
   const Offs o = offsets(d, x, y, z);
   R f[K::Q];
-  load_node<K, P, NT_LD>(A, a.stride, n, o, a.inv_scale, f);
-  (void)collide_fused<K>(f, p);
-  store_node<K, P, NT_ST>(A, a.stride, n, o, a.scale, f);
+  if constexpr (K::C::scaled) {
+    // 16-bit storage: the node body runs in the persistent loop, where the
+    // direction bases and the 27 scales would otherwise be held in scalar
+    // registers across every iteration (and spilled through vector lanes).
+    // The empty asm statements make the stride and a zero index opaque, so
+    // that bases and scales are formed again, from the kernel arguments, for
+    // the loads and, after the collision, for the stores; 1/s comes from
+    // the bits of the power-of-two scale.
+    uint64_t stride_ld = a.stride;
+    uint32_t k0 = 0;
+    asm volatile("" : "+s"(stride_ld), "+s"(k0));
+    float inv[K::Q];
+    for (int q = 0; q < K::Q; ++q)
+      inv[q] = __builtin_bit_cast(float,
+                                  0x7F000000u - __builtin_bit_cast(uint32_t, a.scale[k0 + q]));
+    load_node<K, P, NT_LD>(A, stride_ld, n, o, inv, f);
+    (void)collide_fused<K>(f, p);
+    uint64_t stride_st = a.stride;
+    uint32_t k1 = 0;
+    asm volatile("" : "+s"(stride_st), "+s"(k1) : "v"(f[0]));
+    store_node<K, P, NT_ST>(A, stride_st, n, o, a.scale + k1, f);
+  } else {
+    load_node<K, P, NT_LD>(A, a.stride, n, o, a.inv_scale, f);
+    (void)collide_fused<K>(f, p);
+    store_node<K, P, NT_ST>(A, a.stride, n, o, a.scale, f);
+  }
 }
 
 // Launch 1 of a pair.  P is the parity of step t.

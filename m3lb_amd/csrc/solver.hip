@@ -730,9 +730,10 @@ int pair_lag(const m3lb_solver* s) {
 // stripes and the lag, and (in AUTO) the populations exceed the last-level
 // cache, where the pass saves HBM traffic, in 4- or 8-byte storage.  Measured
 // at 512^3, D3Q27 CM: FP32 31.7 against 24.5 GLUPS, FP64 14.6 against 12.3;
-// the 16-bit forms, whose step is already close to its arithmetic limit,
-// lose (FP16 45.6 against 49.0, E2M13 38.1 against 48.7), so AUTO leaves
-// them to one launch per step.
+// the 16-bit forms lose at every setting of stripes (8, 16), blocks per CU
+// (4, 6, 7) and tiles per ticket (1, 2, 4): at the best, FP16 2.92 against
+// 2.80 ms per step and E2M13 2.98 against 2.78 (at the defaults 3.59 and
+// 3.94), so AUTO leaves them to one launch per step.
 bool pair_eligible(const m3lb_solver* s, const m3lb::StepArgs& a) {
   if (!s->ks->pair || s->fusion == M3LB_FUSION_OFF) return false;
   if (s->b.nwall || s->b.nspec || s->b.nsnap) return false;

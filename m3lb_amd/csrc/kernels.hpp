@@ -324,8 +324,9 @@ __device__ __forceinline__ void node_update(typename K::T* __restrict__ A, uint6
     load_node<K, P, NTL>(A, stride, n, o, inv_scale, f);
   }
   const Macro<R> m = collide_fused<K>(f, p);
-  if constexpr (K::C::scaled) asm volatile("" : "+s"(stride_st) : "v"(f[0]));
-  store_node<K, P, NTL>(A, stride_st, n, o, scale, f);
+  uint32_t k1 = 0;
+  if constexpr (K::C::scaled) asm volatile("" : "+s"(stride_st), "+s"(k1) : "v"(f[0]));
+  store_node<K, P, NTL>(A, stride_st, n, o, scale + k1, f);
   if (macro && macro[0]) {
     static_cast<R*>(macro[0])[n] = density<K>(m);
     static_cast<R*>(macro[1])[n] = m.ux;
@@ -450,12 +451,26 @@ constexpr int kRunBlockRanges = -2;
 
 // RANGES: the kRunBlockRanges order (a separate instance, so that neither
 // order's loop raises the other's register count).
+// The header and the descriptors do not change during a launch and every
+// index into them is uniform over a block, so they are read through the
+// constant address space: scalar loads, and the box's pointer, stride and
+// scales in scalar registers (read through generic pointers they were vector
+// loads, the slots were addressed by 64-bit flat instructions and every
+// scale constant took vector instructions).
+using ConstBoxDesc = const __attribute__((address_space(4))) BoxDesc;
+using ConstWord = const __attribute__((address_space(4))) unsigned;
+
+__device__ __forceinline__ const BoxDesc& box_at(const BoxDesc* desc, unsigned i) {
+  return *(const BoxDesc*)((ConstBoxDesc*)desc + i);
+}
+
 template <class K, bool RANGES, bool NTL>
 __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
-  const unsigned nbox = a.hdr[0];
-  const unsigned* start = a.hdr + 2;  // start[nbox] == all tiles of the chunk
+  ConstWord* hdr = (ConstWord*)a.hdr;
+  const unsigned nbox = hdr[0];
+  ConstWord* start = hdr + 2;  // start[nbox] == all tiles of the chunk
   // only the tiles of boxes that step at j: the list's first `total`
-  const unsigned total = a.hdr[3 + nbox + unsigned(a.j)];
+  const unsigned total = hdr[3 + nbox + unsigned(a.j)];
   if constexpr (RANGES) {
     const unsigned per = (total + gridDim.x - 1) / gridDim.x;
     unsigned t = blockIdx.x * per;
@@ -469,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
     }
     while (t < end) {
       while (start[lo + 1] <= t) ++lo;
-      const BoxDesc& b = a.desc[lo];
+      const BoxDesc& b = box_at(a.desc, lo);
       const unsigned stop = min(end, start[lo + 1]);
       for (; t < stop; ++t) batched_tile<K, NTL>(b, t - start[lo], a.j);
     }
@@ -483,7 +498,7 @@ __global__ void __launch_bounds__(kBlock) k_step_batched(BatchArgs a) {
       if (start[mid] <= t) lo = mid;
       else hi = mid;
     }
-    batched_tile<K, NTL>(a.desc[lo], t - start[lo], a.j);
+    batched_tile<K, NTL>(box_at(a.desc, lo), t - start[lo], a.j);
   }
 }
 

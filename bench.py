@@ -393,7 +393,8 @@ def main():
                     help="requests of the batched-serving leg (0 skips it)")
     ap.add_argument("--serving-rate", type=float, default=40.0, help="arrivals per second")
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
-    ap.add_argument("--serving-chunk", type=int, default=8, help="steps per chunk")
+    ap.add_argument("--serving-chunk", type=int, default=32,
+                    help="steps per chunk (32: best of 4, 8, 16 and 32 on both serving samples)")
     ap.add_argument("--serving-tune-steps", type=int, default=3)
     ap.add_argument("--serving-cap", type=int, default=2400000,
                     help="max resident interior nodes of the server (0: memory only); "

@@ -304,7 +304,7 @@ int m3lb_time_steps(m3lb_solver* s, int32_t nsteps, double* ms);
 /* Order in which the step kernel's tiles are assigned to the GPU's 8 XCDs:
  * 0 = hardware order, -1 = each XCD takes one contiguous eighth of the box,
  * run > 0 = XCDs take turns with runs of `run` consecutive tiles (default 64
- * for 4- and 8-byte populations, 2 for 16-bit storage).
+ * for 4- and 8-byte populations, 8 for E2M13, 2 for the other 16-bit forms).
  * A tuning knob only: results are bit-identical for every value.  No
  * counterpart in the reference. */
 int m3lb_set_tile_order(m3lb_solver* s, int32_t run);

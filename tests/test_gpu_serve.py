@@ -170,6 +170,21 @@ def test_graph_and_eager_agree():
         assert np.array_equal(g[k][0], e[k][0]), k
 
 
+def test_long_chunks_match_single_solver():
+    """Chunks of 32 steps (the length the serving bench uses), each one
+    captured graph of 32 launches: boxes whose step counts are not multiples
+    of 32 stop inside a chunk, a box joins mid-run, and every box matches the
+    single-box solver bit for bit."""
+    scfg = abi.make_server_config(budget_bytes=512 * MiB, steps_per_chunk=32)
+    mks = [lambda: tg((32, 24, 20), 37), lambda: channel((28, 30, 18), 70),
+           lambda: tg((300, 9, 7), 33), lambda: tg((45, 17, 19), 100)]
+    got = run_server(scfg, [mk() for mk in mks], between=1)
+    for k, mk in enumerate(mks):
+        pops, mass, _ = solo(scfg, mk())
+        assert np.array_equal(got[k][0], pops), k
+        assert abs(got[k][1]["mass"] - mass) <= 1e-12 * abs(mass), k
+
+
 def test_budget_queues_and_rejects():
     scfg = abi.make_server_config(budget_bytes=64 * MiB, steps_per_chunk=2)
     mk = lambda: tg((40, 40, 40), 6)  # 7 MB of FP32 populations
